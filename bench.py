@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Throughput of the Greenstein-Winslow coupled integrator on one MI355X.
+
+Workload: 64 independent seeds (64 cells) of a 12,500-CRU cell, plain FP64,
+1 Hz pacing with a +100 pA/pF, 1 ms stimulus (the reference's sign: positive is
+depolarising), dt = 1e-3 ms, starting from the seeded rest state. One bench
+step is one Euler step of the whole batch (every cell: global ODE + SSA of all
+its CRUs). W warmup steps run untimed first, then exactly K steps are timed
+with HIP events on the library's stream.
+
+  value        CRU*ms simulated per second over the whole batch
+  cell_beats   cells * (simulated ms / 1000 ms per beat) per second
+  roofline     algorithmic FP64 flops (F_ITER per SSA iteration x iterations
+               counted in-kernel) per launch / launch time, against the FP64
+               vector peak; state bytes per launch against HBM
+  cpu_baseline the CPU restatement built with the reference's release flags,
+               OpenMP over CRUs on all host cores, on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gw-model_amd")
+
+F_ITER = 330            # FP64 flops of one SSA loop iteration (count in DESIGN.md)
+STATE_BYTES = 280       # bytes read + written per CRU-step by the FP64 SSA kernel (layout in DESIGN.md)
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def load_capi():
+    spec = importlib.util.spec_from_file_location("gw_capi_binding", os.path.join(PKG, "capi.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pacing(capi):
+    return {"kind": capi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+
+
+def gpu_bench(capi, cells, crus, steps, warmup, seed0=42):
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, [capi.make_cell(crus, seed0 + c, None, pacing(capi)) for c in range(cells)])
+    b.set_mode(frozen=False, digest=False)
+    if warmup > 0:
+        b.step(warmup)
+    i0 = b.iterations()
+    ms, launches = b.time_steps(steps)
+    iters = b.iterations() - i0
+    g = b.get_globals()
+    obs = b.obs()
+    b.close()
+    ctx.close()
+    return ms, launches, iters, g, obs
+
+
+def cpu_baseline(crus, steps, warmup_steps=0, seed=42):
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import OracleModel, lib
+
+    L = lib("omp")
+    threads = L.gwo_model_threads()
+    m = OracleModel(crus, seed=seed, kind="omp")
+    m.sample_rest_state()
+    ist = np.zeros(steps + warmup_steps)
+    t = 0.0
+    for k in range(len(ist)):
+        ist[k] = 100.0 if t < 1.0 else 0.0
+        t += 1e-3
+    if warmup_steps:
+        m.run(1e-3, warmup_steps, ist[:warmup_steps])
+    t0 = time.perf_counter()
+    rc = L.gwo_model_run(m.h, 1e-3, steps, ist[warmup_steps:].ctypes.data_as(C.c_void_p), 0, None)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed: {rc}")
+    return crus * steps * 1e-3 / dt, threads, dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000, help="timed Euler steps (dt = 1e-3 ms)")
+    ap.add_argument("--warmup", type=int, default=1000, help="untimed Euler steps before the timed region")
+    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--crus", type=int, default=12500)
+    ap.add_argument("--cpu-crus", type=int, default=12500)
+    ap.add_argument("--cpu-steps", type=int, default=20000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(cells are independent and would shard by cell, see DESIGN.md)", file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+    capi = load_capi()
+    ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup)
+    total_crus = args.cells * args.crus
+    sim_ms = args.steps * 1e-3
+    sec = ms / 1e3
+    value = total_crus * sim_ms / sec
+    cell_beats = args.cells * (sim_ms / 1000.0) / sec
+    per_launch_s = sec / launches
+    flops_per_launch = F_ITER * iters / launches
+    achieved_tflops = flops_per_launch / per_launch_s / 1e12
+    bytes_per_launch = STATE_BYTES * total_crus
+    achieved_gbs = bytes_per_launch / per_launch_s / 1e9
+    frac_f = achieved_tflops / PEAK_FP64_TFLOPS
+    frac_b = achieved_gbs / PEAK_HBM_GBS
+    line = {
+        "metric": "CRU*ms simulated/s and cell-beats/s, batched FP32-local vs plain FP64, 1 GPU",
+        "value": value,
+        "unit": "CRU*ms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded rest state, 1 Hz pacing",
+        "config": {"workload": f"plain FP64, {args.cells} cells (independent seeds) x {args.crus} CRUs, "
+                               f"1 Hz pacing, dt 1e-3 ms, timed window t = {args.warmup * 1e-3:g}.."
+                               f"{(args.warmup + args.steps) * 1e-3:g} ms of beat 1",
+                   "cells": args.cells, "crus_per_cell": args.crus, "dt_ms": 1e-3, "precision": "fp64"},
+        "cell_beats_per_s": cell_beats,
+        "iterations_per_cru_step": iters / (launches * total_crus),
+        "roofline": {
+            "bound": "valu" if frac_f >= frac_b else "hbm",
+            "achieved": achieved_tflops if frac_f >= frac_b else achieved_gbs,
+            "peak": PEAK_FP64_TFLOPS if frac_f >= frac_b else PEAK_HBM_GBS,
+            "unit": "TFLOP/s" if frac_f >= frac_b else "GB/s",
+            "frac": max(frac_f, frac_b),
+            "traffic": None,
+            "kernel": "k_ssa_step<0,false>",
+            "avg_launch_us": per_launch_s * 1e6,
+            "fp64": {"achieved_tflops": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "frac": frac_f,
+                     "flops_per_launch": flops_per_launch, "f_iter": F_ITER},
+            "hbm": {"achieved_gbs": achieved_gbs, "peak": PEAK_HBM_GBS, "frac": frac_b,
+                    "bytes_per_launch": bytes_per_launch},
+        },
+        "mean_V_end": float(np.mean([x.V for x in g])),
+    }
+    if not args.no_cpu and rank == 0:
+        v, threads, secs = cpu_baseline(args.cpu_crus, args.cpu_steps)
+        line["cpu_baseline"] = {"value": v, "unit": "CRU*ms/s", "cores": threads, "kind": "port",
+                                "sample": f"1 cell x {args.cpu_crus} CRUs x {args.cpu_steps} steps "
+                                          f"(t = 0..{args.cpu_steps * 1e-3:g} ms of beat 1), {secs:.1f} s",
+                                "ratio_gpu_over_cpu": value / v}
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1399 @@
+// gw_capi.hip -- kernels of the MI355X Greenstein-Winslow integrator and the
+// C-ABI of include/gw_capi.h.
+//
+// Device layout: every CRU of every cell owns one slot of structure-of-arrays
+// state (CaSS[4][ntot], CaJSR, JLCC[4][ntot], packed channel word, four packed
+// RyR count words, four engine words). Each cell's slots are padded to whole
+// 256-thread blocks, so a block belongs to exactly one cell and reads that
+// cell's constants once.
+//
+// One Euler step (GW_model::euler_step, src/includes/GW.tpp:236-272) is one
+// launch of k_ssa_step: every block advances its CRUs through the SSA with the
+// cell's frozen constants, reduces the flux sums exactly (128-bit fixed point)
+// into a block partial, and the last block of the cell to finish (an agent-scope
+// acq_rel counter) forms the cell's sums, applies globals += dGlobals and
+// prepares the next step: derivatives at the new state and the SSA constants.
+#include "gw_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace gw;
+
+namespace {
+
+// ===========================================================================
+// device helpers
+// ===========================================================================
+__device__ double protocol_istim(const DevCell &C, const CellDyn &d)
+{
+    switch (C.proto.kind) {
+    case GW_PROTO_TABLE:
+        return (C.istim && d.step < C.istim_len) ? C.istim[d.step] : 0.0;
+    case GW_PROTO_PACE: {
+        const double x = d.t - C.proto.start;
+        if (x < 0.0) return 0.0;
+        const double ph = (C.proto.period > 0.0 && C.proto.period < 1e300) ? fmod(x, C.proto.period) : x;
+        return ph < C.proto.duration ? C.proto.amplitude : 0.0;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+__device__ double protocol_vclamp(const DevCell &C, double t)
+{
+    return (t >= C.proto.t_on && t < C.proto.t_off) ? C.proto.step : C.proto.hold;
+}
+
+__device__ void obs_update(gw_cell_obs_t &o, double V_old, double V, double t_old, double dt, double Cai, double ICaL,
+                           double open_frac, double jrel_scaled)
+{
+    o.peak_V = fmax(o.peak_V, V);
+    o.min_V = fmin(o.min_V, V);
+    o.peak_Cai = fmax(o.peak_Cai, Cai);
+    o.peak_ICaL_neg = fmin(o.peak_ICaL_neg, ICaL);
+    o.sum_open_RyR_frac += open_frac;
+    o.int_Jrel += jrel_scaled * dt;
+    o.n_steps += 1.0;
+    // APD90: upstroke = upward crossing of -40 mV; end = downward crossing of
+    // V_rest + 0.1 (V_peak - V_rest), both linearly interpolated between steps.
+    if (o.apd_t_up < 0.0) {
+        o.apd_v_rest = fmin(o.apd_v_rest, V);
+        if (V_old < -40.0 && V >= -40.0) {
+            o.apd_t_up = t_old + dt * (-40.0 - V_old) / (V - V_old);
+            o.apd_v_peak = V;
+        }
+    } else {
+        o.apd_v_peak = fmax(o.apd_v_peak, V);
+        const double thr = o.apd_v_rest + 0.1 * (o.apd_v_peak - o.apd_v_rest);
+        if (V_old > thr && V <= thr && V < o.apd_v_peak - 10.0) {
+            const double tx = t_old + dt * (V_old - thr) / (V_old - V);
+            const double apd = tx - o.apd_t_up;
+            o.apd90_prev = o.apd90_last;
+            o.apd90_last = apd;
+            if (o.beats_done >= 1.0) {
+                o.alt_sum += fabs(apd - o.apd90_prev);
+                o.alt_count += 1.0;
+            }
+            o.beats_done += 1.0;
+            o.apd_t_up = -1.0;
+            o.apd_v_rest = V;
+        }
+    }
+    o.v_prev = V;
+}
+
+// Derivatives and SSA constants of the step d.step at state d.g (the part of
+// euler_step before SSA(dt), GW.tpp:238-247 and 216-221).
+__device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, double dt, Currents &cur)
+{
+    d.Istim = protocol_istim(C, d);
+    if (d.clamp) d.g.V = protocol_vclamp(C, d.t);
+    const double VF_RT = d.g.V * C.c.F_RT;
+    const double expm = gw::exp(-VF_RT);
+    QMatrices Q;
+    q_matrices(Q, C.p, d.g.V);
+    const int rc = global_derivatives(C.p, C.c, d.g, VF_RT, expm, dt, d.Istim, S, d.dg, cur, Q);
+    if (rc != GW_OK && d.status == 0) d.status = rc;
+    step_consts(d.k, C.p, C.c, d.g.V, expm, d.g.Cai, d.g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+}
+
+__device__ void sum_block_partials(const Partial *parts, int64_t first, int n, Partial &tot)
+{
+    fx_zero(tot.jlcc); fx_zero(tot.jxfer); fx_zero(tot.jtr); fx_zero(tot.jrel);
+    tot.clch = 0; tot.open = 0; tot.iters = 0; tot.err = 0;
+    for (int b = 0; b < n; ++b) {
+        const Partial &q = parts[first + b];
+        fx_add(tot.jlcc, q.jlcc);
+        fx_add(tot.jxfer, q.jxfer);
+        fx_add(tot.jtr, q.jtr);
+        fx_add(tot.jrel, q.jrel);
+        tot.clch += q.clch;
+        tot.open += q.open;
+        tot.iters += q.iters;
+        tot.err |= q.err;
+    }
+}
+
+__device__ void partial_add(Partial &a, const Partial &b)
+{
+    fx_add(a.jlcc, b.jlcc);
+    fx_add(a.jxfer, b.jxfer);
+    fx_add(a.jtr, b.jtr);
+    fx_add(a.jrel, b.jrel);
+    a.clch += b.clch;
+    a.open += b.open;
+    a.iters += b.iters;
+    a.err |= b.err;
+}
+
+__device__ void block_reduce(Partial *red, Partial mine)
+{
+    const int tid = threadIdx.x;
+    red[tid] = mine;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) partial_add(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
+// kernels
+// ===========================================================================
+
+// Seed the per-CRU engines and replay digests; zero the padding slots.
+__global__ void k_seed(KArgs a)
+{
+    const int b = blockIdx.x;
+    const int cell = a.block_cell[b];
+    const DevCell &C = a.cells[cell];
+    const int64_t i = (b - C.blk0) * kBlock + threadIdx.x;
+    const int64_t slot = C.slot0 + i;
+    uint64_t s[4];
+    rng_seed(a.prng, s, stream_key(C.seed, i));
+    for (int k = 0; k < 4; ++k) a.rng[k * a.ntot + slot] = s[k];
+    if (a.dig) {
+        a.dig[slot] = 0;
+        a.dig[a.ntot + slot] = 0xcbf29ce484222325ULL;
+    }
+    if (i >= C.n_cru) {  // padding: a valid, inert state
+        for (int j = 0; j < 4; ++j) {
+            a.CaSS[j * a.ntot + slot] = 0.0;
+            a.JLCC[j * a.ntot + slot] = 0.0;
+            a.ryr[j * a.ntot + slot] = 0;
+        }
+        a.CaJSR[slot] = 0.0;
+        a.chan[slot] = 0x1111u;
+    }
+}
+
+// Rest-state sampler (initialise_LCC/_LCC_i/_RyR/_ClCh, src/includes/GW_utils.tpp:5-64,
+// and the CRUState constants, GW_utils.tpp:212-213) on a per-CRU stream keyed
+// by the complement of the CRU's SSA stream key.
+template <int PRNG>
+__global__ void k_sample_rest(KArgs a)
+{
+    const int b = blockIdx.x;
+    const int cell = a.block_cell[b];
+    const DevCell &C = a.cells[cell];
+    const int64_t i = (b - C.blk0) * kBlock + threadIdx.x;
+    if (i >= C.n_cru) return;
+    const int64_t slot = C.slot0 + i;
+    double ca[4], jsr;
+    uint32_t chan;
+    uint64_t ryr[4];
+    sample_rest_cru<PRNG>(C.seed, i, ca, jsr, chan, ryr);
+    for (int j = 0; j < 4; ++j) {
+        a.CaSS[j * a.ntot + slot] = ca[j];
+        a.JLCC[j * a.ntot + slot] = 0.0;
+        a.ryr[j * a.ntot + slot] = ryr[j];
+    }
+    a.CaJSR[slot] = jsr;
+    a.chan[slot] = chan;
+}
+
+// Step-0 preparation of every cell (one block per cell): the flux sums the
+// reference's first euler_step reads -- JLCC of the given state, Jxfer and Jtr
+// of the constructor's default CRU state and default globals (GW.hpp:104-113),
+// ClCh of the given state -- then derivatives and SSA constants.
+__global__ void k_init_cells(KArgs a)
+{
+    __shared__ Partial red[kBlock];
+    const int cell = blockIdx.x;
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    Partial mine;
+    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
+    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    const double X = C.p.rxfer * (1.45370e-4 - 1.45273e-4);
+    const double X4 = X + X + X + X;
+    const double Y = C.p.rtr * (0.908882 - 0.908408);
+    for (int64_t i = threadIdx.x; i < C.n_cru; i += kBlock) {
+        const int64_t slot = C.slot0 + i;
+        double jl[4];
+        for (int j = 0; j < 4; ++j) jl[j] = a.JLCC[j * a.ntot + slot];
+        mine.err |= fx_add(mine.jlcc, jl[0] + jl[1] + jl[2] + jl[3]);
+        mine.err |= fx_add(mine.jxfer, X4);
+        mine.err |= fx_add(mine.jtr, Y);
+        const uint32_t ch = a.chan[slot];
+        for (int j = 0; j < 4; ++j) {
+            mine.clch += chan_cl(ch, j);
+            const uint64_t w = a.ryr[j * a.ntot + slot];
+            mine.open += ryr_get(w, 2) + ryr_get(w, 3);
+        }
+    }
+    block_reduce(red, mine);
+    if (threadIdx.x == 0) {
+        const Partial &t = red[0];
+        FluxSums S{fx_to_double(t.jlcc), fx_to_double(t.jxfer), fx_to_double(t.jtr), static_cast<double>(t.clch)};
+        D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
+        D.sums[4] = static_cast<double>(t.open); D.sums[5] = 0.0;
+        D.t = 0.0;
+        D.step = 0;
+        D.status = t.err ? GW_E_RANGE : 0;
+        Currents cur;
+        prepare_step(C, D, S, a.dt, cur);
+        D.obs.v_prev = D.g.V;
+        a.counters[cell] = 0;
+    }
+}
+
+// One Euler step of every cell.
+template <int PRNG, bool DIGEST>
+__global__ __launch_bounds__(kBlock) void k_ssa_step(KArgs a)
+{
+    __shared__ SsaParams sp;
+    __shared__ StepConsts kk;
+    __shared__ Partial red[kBlock];
+    __shared__ int is_last;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int cell = a.block_cell[b];
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    {
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(&C.sp);
+        uint64_t *dst = reinterpret_cast<uint64_t *>(&sp);
+        for (int w = tid; w < static_cast<int>(sizeof(SsaParams) / 8); w += kBlock) dst[w] = src[w];
+        const uint64_t *src2 = reinterpret_cast<const uint64_t *>(&D.k);
+        uint64_t *dst2 = reinterpret_cast<uint64_t *>(&kk);
+        for (int w = tid; w < static_cast<int>(sizeof(StepConsts) / 8); w += kBlock) dst2[w] = src2[w];
+    }
+    __syncthreads();
+    const int64_t i = (b - C.blk0) * kBlock + tid;
+    const bool active = (i < C.n_cru) && (D.status == 0);
+    Partial mine;
+    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
+    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    if (active) {
+        const int64_t slot = C.slot0 + i;
+        const int64_t N = a.ntot;
+        Cru64 c;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c.ca[j] = a.CaSS[j * N + slot];
+            c.jl[j] = a.JLCC[j * N + slot];
+            c.ryr[j] = a.ryr[j * N + slot];
+        }
+        c.jsr = a.CaJSR[slot];
+        c.chan = a.chan[slot];
+        c.r0 = a.rng[slot];
+        c.r1 = a.rng[N + slot];
+        c.r2 = a.rng[2 * N + slot];
+        c.r3 = a.rng[3 * N + slot];
+        Digest dg{0, 0};
+        if (DIGEST) {
+            dg.count = a.dig[slot];
+            dg.hash = a.dig[N + slot];
+        }
+        const CruOut o = ssa_step<PRNG, DIGEST>(c, sp, kk, a.dt, dg);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a.CaSS[j * N + slot] = c.ca[j];
+            a.JLCC[j * N + slot] = c.jl[j];
+            a.ryr[j * N + slot] = c.ryr[j];
+        }
+        a.CaJSR[slot] = c.jsr;
+        a.chan[slot] = c.chan;
+        a.rng[slot] = c.r0;
+        a.rng[N + slot] = c.r1;
+        a.rng[2 * N + slot] = c.r2;
+        a.rng[3 * N + slot] = c.r3;
+        if (DIGEST) {
+            a.dig[slot] = dg.count;
+            a.dig[N + slot] = dg.hash;
+        }
+        mine.err |= fx_add(mine.jlcc, o.JLCC);
+        mine.err |= fx_add(mine.jxfer, o.Jxfer);
+        mine.err |= fx_add(mine.jtr, o.Jtr);
+        fx_add(mine.jrel, o.Jrel);
+        mine.clch = o.clch;
+        mine.open = o.open;
+        mine.iters = o.iters;
+        mine.err |= o.err ? 4 : 0;
+    }
+    block_reduce(red, mine);
+    if (tid == 0) {
+        a.partials[b] = red[0];
+        const uint32_t prev = __hip_atomic_fetch_add(&a.counters[cell], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (prev == static_cast<uint32_t>(C.nblk - 1));
+    }
+    __syncthreads();
+    if (!is_last || tid != 0) return;
+    // ---- last block of the cell: close step n, prepare step n+1 ----
+    Partial tot;
+    sum_block_partials(a.partials, C.blk0, C.nblk, tot);
+    a.counters[cell] = 0;
+    if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
+    if (D.status != 0) return;
+    if (tot.err) {
+        D.status = GW_E_RANGE;
+        return;
+    }
+    const FluxSums S{fx_to_double(tot.jlcc), fx_to_double(tot.jxfer), fx_to_double(tot.jtr), static_cast<double>(tot.clch)};
+    const double jrel = fx_to_double(tot.jrel);
+    D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
+    D.sums[4] = static_cast<double>(tot.open); D.sums[5] = jrel;
+    const double V_old = D.g.V;
+    const double t_old = D.t;
+    if (!a.frozen) globals_apply(D.g, D.dg, D.clamp != 0);
+    D.t += a.dt;
+    D.step += 1;
+    Currents cur;
+    prepare_step(C, D, S, a.dt, cur);
+    if (!(D.g.V == D.g.V) || !(D.g.Cai == D.g.Cai)) {
+        if (D.status == 0) D.status = GW_E_NAN;
+    }
+    const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
+    obs_update(D.obs, V_old, D.g.V, t_old, a.dt, D.g.Cai, cur.ICaL, open_frac, jrel * C.c.CRU_factor);
+    if (a.trace_every > 0 && (D.step % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
+        double *row = a.trace + (D.trace_rows * a.ncells + cell) * 4;
+        row[0] = D.g.V;
+        row[1] = D.g.Cai;
+        row[2] = cur.ICaL;
+        row[3] = open_frac;
+        D.trace_rows += 1;
+    }
+}
+
+// Frozen-globals replay: every lane advances its CRU through `nsteps` steps
+// with the cell's constant SSA constants (globals held fixed), so only the
+// SSA runs. The flux sums of the final step are reduced as in k_ssa_step.
+template <int PRNG, bool DIGEST>
+__global__ __launch_bounds__(kBlock) void k_ssa_frozen(KArgs a, int64_t step0, int64_t nsteps)
+{
+    __shared__ SsaParams sp;
+    __shared__ StepConsts kk;
+    __shared__ Partial red[kBlock];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int cell = a.block_cell[b];
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    {
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(&C.sp);
+        uint64_t *dst = reinterpret_cast<uint64_t *>(&sp);
+        for (int w = tid; w < static_cast<int>(sizeof(SsaParams) / 8); w += kBlock) dst[w] = src[w];
+        const uint64_t *src2 = reinterpret_cast<const uint64_t *>(&D.k);
+        uint64_t *dst2 = reinterpret_cast<uint64_t *>(&kk);
+        for (int w = tid; w < static_cast<int>(sizeof(StepConsts) / 8); w += kBlock) dst2[w] = src2[w];
+    }
+    __syncthreads();
+    const int64_t i = (b - C.blk0) * kBlock + tid;
+    const bool active = (i < C.n_cru) && (D.status == 0);
+    Partial mine;
+    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
+    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    if (active) {
+        const int64_t slot = C.slot0 + i;
+        const int64_t N = a.ntot;
+        Cru64 c;
+        for (int j = 0; j < 4; ++j) {
+            c.ca[j] = a.CaSS[j * N + slot];
+            c.jl[j] = a.JLCC[j * N + slot];
+            c.ryr[j] = a.ryr[j * N + slot];
+        }
+        c.jsr = a.CaJSR[slot];
+        c.chan = a.chan[slot];
+        c.r0 = a.rng[slot];
+        c.r1 = a.rng[N + slot];
+        c.r2 = a.rng[2 * N + slot];
+        c.r3 = a.rng[3 * N + slot];
+        Digest dg{0, 0};
+        if (DIGEST) {
+            dg.count = a.dig[slot];
+            dg.hash = a.dig[N + slot];
+        }
+        StepConsts k = kk;
+        CruOut o{};
+        uint64_t iters = 0;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            k.step_lo = static_cast<uint32_t>((step0 + s) & 0xffff);
+            o = ssa_step<PRNG, DIGEST>(c, sp, k, a.dt, dg);
+            iters += o.iters;
+            if (o.err) break;
+        }
+        for (int j = 0; j < 4; ++j) {
+            a.CaSS[j * N + slot] = c.ca[j];
+            a.JLCC[j * N + slot] = c.jl[j];
+            a.ryr[j * N + slot] = c.ryr[j];
+        }
+        a.CaJSR[slot] = c.jsr;
+        a.chan[slot] = c.chan;
+        a.rng[slot] = c.r0;
+        a.rng[N + slot] = c.r1;
+        a.rng[2 * N + slot] = c.r2;
+        a.rng[3 * N + slot] = c.r3;
+        if (DIGEST) {
+            a.dig[slot] = dg.count;
+            a.dig[N + slot] = dg.hash;
+        }
+        mine.err |= fx_add(mine.jlcc, o.JLCC);
+        mine.err |= fx_add(mine.jxfer, o.Jxfer);
+        mine.err |= fx_add(mine.jtr, o.Jtr);
+        fx_add(mine.jrel, o.Jrel);
+        mine.clch = o.clch;
+        mine.open = o.open;
+        mine.iters = iters;
+        mine.err |= o.err ? 4 : 0;
+    }
+    block_reduce(red, mine);
+    if (tid == 0) a.partials[b] = red[0];
+}
+
+// Close a frozen replay of nsteps: sums of the last step, clock advanced as
+// t += dt per step (src/pyGW.hpp:116).
+__global__ void k_frozen_close(KArgs a, int64_t nsteps)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncells) return;
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    Partial tot;
+    sum_block_partials(a.partials, C.blk0, C.nblk, tot);
+    if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
+    if (tot.err && D.status == 0) D.status = GW_E_RANGE;
+    D.sums[0] = fx_to_double(tot.jlcc); D.sums[1] = fx_to_double(tot.jxfer); D.sums[2] = fx_to_double(tot.jtr);
+    D.sums[3] = static_cast<double>(tot.clch); D.sums[4] = static_cast<double>(tot.open); D.sums[5] = fx_to_double(tot.jrel);
+    for (int64_t s = 0; s < nsteps; ++s) D.t += a.dt;
+    D.step += nsteps;
+    D.k.step_lo = static_cast<uint32_t>(D.step & 0xffff);
+}
+
+// SoA <-> row-major host layout of one cell (GWCRUState, src/pyGW.cpp:78-87).
+struct StateView {
+    double *CaSS, *CaJSR, *JLCC;
+    int32_t *LCC, *LCCi, *RyR, *ClCh;
+};
+
+__global__ void k_put_state(KArgs a, int cell, StateView v, int has_jlcc)
+{
+    const DevCell &C = a.cells[cell];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= C.n_cru) return;
+    const int64_t slot = C.slot0 + i;
+    uint32_t chan = 0;
+    for (int j = 0; j < 4; ++j) {
+        a.CaSS[j * a.ntot + slot] = v.CaSS[i * 4 + j];
+        a.JLCC[j * a.ntot + slot] = has_jlcc ? v.JLCC[i * 4 + j] : 0.0;
+        chan = chan_set_lcc(chan, j, v.LCC[i * 4 + j]);
+        if (v.LCCi[i * 4 + j]) chan |= 1u << (16 + j);
+        if (v.ClCh[i * 4 + j]) chan |= 1u << (20 + j);
+        uint64_t w = 0;
+        for (int k = 0; k < 6; ++k) w = ryr_set(w, k, v.RyR[(i * 4 + j) * 6 + k]);
+        a.ryr[j * a.ntot + slot] = w;
+    }
+    a.CaJSR[slot] = v.CaJSR[i];
+    a.chan[slot] = chan;
+}
+
+__global__ void k_get_state(KArgs a, int cell, StateView v)
+{
+    const DevCell &C = a.cells[cell];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= C.n_cru) return;
+    const int64_t slot = C.slot0 + i;
+    const uint32_t chan = a.chan[slot];
+    for (int j = 0; j < 4; ++j) {
+        v.CaSS[i * 4 + j] = a.CaSS[j * a.ntot + slot];
+        v.JLCC[i * 4 + j] = a.JLCC[j * a.ntot + slot];
+        v.LCC[i * 4 + j] = chan_lcc(chan, j);
+        v.LCCi[i * 4 + j] = chan_y(chan, j);
+        v.ClCh[i * 4 + j] = chan_cl(chan, j);
+        const uint64_t w = a.ryr[j * a.ntot + slot];
+        for (int k = 0; k < 6; ++k) v.RyR[(i * 4 + j) * 6 + k] = ryr_get(w, k);
+    }
+    v.CaJSR[i] = a.CaJSR[slot];
+}
+
+__global__ void k_reset_obs(CellDyn *dyn, int ncells)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    gw_cell_obs_t &o = dyn[c].obs;
+    const double V = dyn[c].g.V;
+    o.peak_V = -1e300; o.min_V = 1e300; o.peak_Cai = -1e300; o.peak_ICaL_neg = 1e300;
+    o.sum_open_RyR_frac = 0; o.int_Jrel = 0; o.n_steps = 0;
+    o.v_prev = V; o.apd_v_rest = V; o.apd_v_peak = V; o.apd_t_up = -1.0;
+    o.apd90_last = -1.0; o.apd90_prev = -1.0; o.alt_sum = 0; o.alt_count = 0; o.beats_done = 0;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+void set_err(char *err, size_t errlen, const char *fmt, ...)
+{
+    if (!err || errlen == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, errlen, fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            set_err(err, errlen, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GW_E_HIP;                                                                        \
+        }                                                                                           \
+    } while (0)
+
+gw_params_t default_params()
+{
+    gw_params_t p;
+#define GW_SET_DEFAULT_(type, name, dflt) p.name = static_cast<type>(dflt);
+    GW_PARAMETER_FIELDS(GW_SET_DEFAULT_)
+#undef GW_SET_DEFAULT_
+    return p;
+}
+
+gw_globals_t default_globals()
+{
+    gw_globals_t g;
+    g.V = -91.382; g.Nai = 10.0; g.Ki = 131.84; g.Cai = 1.45273e-4; g.CaNSR = 0.908882;
+    g.CaLTRPN = 8.98282e-3; g.CaHTRPN = 0.137617; g.m = 5.33837e-4; g.h = 0.996345; g.j = 0.997315;
+    g.xKs = 2.04171e-4;
+    const double Kr[5] = {0.999503, 4.13720e-4, 7.27568e-5, 8.73984e-6, 1.36159e-6};
+    const double Kv43[10] = {0.953060, 0.0253906, 2.53848e-4, 1.12796e-6, 1.87950e-9,
+                             0.0151370, 0.00517622, 8.96600e-4, 8.17569e-5, 2.24032e-6};
+    const double Kv14[10] = {0.722328, 0.101971, 0.00539932, 1.27081e-4, 1.82742e-6,
+                             0.152769, 0.00962328, 0.00439043, 0.00195348, 0.00143629};
+    for (int i = 0; i < 5; ++i) g.Kr[i] = Kr[i];
+    for (int i = 0; i < 10; ++i) { g.Kv43[i] = Kv43[i]; g.Kv14[i] = Kv14[i]; }
+    return g;
+}
+
+}  // namespace
+
+struct gw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<gw_batch *> batches;  // live batches; released by gw_ctx_destroy
+};
+
+struct gw_batch {
+    gw_ctx *ctx = nullptr;
+    int32_t ncells = 0, precision = 0, prng = 0;
+    double dt = 1e-3;
+    int64_t ntot = 0, nblocks = 0;
+    bool digest = true;
+    bool frozen = false;
+    bool prepared = false;
+    int64_t steps_done = 0;
+    int32_t trace_every = 0;
+    int64_t trace_cap = 0;
+    int32_t crus_per_thread = 1, block = kBlock;
+    std::vector<DevCell> cells_h;
+    std::vector<int32_t> block_cell_h;
+    std::vector<gw_globals_t> g0;
+    std::vector<double *> istim_dev;
+    KArgs a{};
+    DevCell *d_cells = nullptr;
+    CellDyn *d_dyn = nullptr;
+    int32_t *d_block_cell = nullptr;
+    void *d_staging = nullptr;
+    size_t staging_bytes = 0;
+};
+
+namespace {
+
+template <class T>
+int dmalloc(T **p, size_t n, char *err, size_t errlen)
+{
+    *p = nullptr;
+    if (n == 0) return GW_OK;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)));
+    return GW_OK;
+}
+
+int validate_params(const gw_params_t &p, char *err, size_t errlen)
+{
+    if (!(p.T > 0 && p.CSA > 0 && p.Vcyto > 0 && p.VNSR > 0 && p.VJSR > 0 && p.VSS > 0 && p.NCaRU > 0 && p.Ko > 0 &&
+          p.Nao > 0 && p.Cao > 0 && p.b > 0)) {
+        set_err(err, errlen, "parameters: temperatures, volumes, NCaRU, b and external concentrations must be > 0");
+        return GW_E_INVALID_ARG;
+    }
+    return GW_OK;
+}
+
+int launch_cfg_check(gw_batch *B, char *err, size_t errlen)
+{
+    if (B->nblocks <= 0 || B->nblocks > (1LL << 31) - 1) {
+        set_err(err, errlen, "invalid number of SSA blocks %lld", static_cast<long long>(B->nblocks));
+        return GW_E_INVALID_ARG;
+    }
+    return GW_OK;
+}
+
+int prepare(gw_batch *B, char *err, size_t errlen)
+{
+    if (B->prepared) return GW_OK;
+    hipLaunchKernelGGL(k_init_cells, dim3(B->ncells), dim3(kBlock), 0, B->ctx->stream, B->a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reset_obs, dim3((B->ncells + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, B->ncells);
+    HIP_TRY(hipGetLastError());
+    B->prepared = true;
+    B->steps_done = 0;
+    return GW_OK;
+}
+
+template <int PRNG>
+void launch_frozen(gw_batch *B, int64_t step0, int64_t n)
+{
+    if (B->digest)
+        hipLaunchKernelGGL((k_ssa_frozen<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+    else
+        hipLaunchKernelGGL((k_ssa_frozen<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+    hipLaunchKernelGGL(k_frozen_close, dim3((B->ncells + 63) / 64), dim3(64), 0, B->ctx->stream, B->a, n);
+}
+
+void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
+{
+    switch (B->prng) {
+    case 0: launch_frozen<0>(B, step0, n); break;
+    case 1: launch_frozen<1>(B, step0, n); break;
+    case 2: launch_frozen<2>(B, step0, n); break;
+    case 3: launch_frozen<3>(B, step0, n); break;
+    case 4: launch_frozen<4>(B, step0, n); break;
+    default: launch_frozen<5>(B, step0, n); break;
+    }
+}
+
+template <int PRNG>
+void launch_step(gw_batch *B)
+{
+    if (B->digest)
+        hipLaunchKernelGGL((k_ssa_step<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
+    else
+        hipLaunchKernelGGL((k_ssa_step<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
+}
+
+void launch_steps(gw_batch *B, int64_t n)
+{
+    for (int64_t s = 0; s < n; ++s) {
+        switch (B->prng) {
+        case 0: launch_step<0>(B); break;
+        case 1: launch_step<1>(B); break;
+        case 2: launch_step<2>(B); break;
+        case 3: launch_step<3>(B); break;
+        case 4: launch_step<4>(B); break;
+        default: launch_step<5>(B); break;
+        }
+    }
+}
+
+int check_status(gw_batch *B, char *err, size_t errlen)
+{
+    for (int c = 0; c < B->ncells; ++c) {
+        int32_t st = 0;
+        HIP_TRY(hipMemcpyAsync(&st, &B->d_dyn[c].status, sizeof(int32_t), hipMemcpyDeviceToHost, B->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+        if (st != 0) {
+            const char *what = st == GW_E_NONPOSITIVE_CONC ? "a Nernst potential received a non-positive concentration"
+                             : st == GW_E_NAN ? "a global state variable became NaN"
+                             : st == GW_E_RANGE ? "a CRU flux or the SSA iteration count left the supported range"
+                                                : "device error";
+            set_err(err, errlen, "cell %d: %s (status %d)", c, what, st);
+            return st;
+        }
+    }
+    return GW_OK;
+}
+
+int sample_rest(gw_batch *B, char *err, size_t errlen)
+{
+    switch (B->prng) {
+    case 0: hipLaunchKernelGGL(k_sample_rest<0>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    case 1: hipLaunchKernelGGL(k_sample_rest<1>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    case 2: hipLaunchKernelGGL(k_sample_rest<2>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    case 3: hipLaunchKernelGGL(k_sample_rest<3>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    case 4: hipLaunchKernelGGL(k_sample_rest<4>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    default: hipLaunchKernelGGL(k_sample_rest<5>, dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return GW_OK;
+}
+
+int ensure_staging(gw_batch *B, size_t bytes, char *err, size_t errlen)
+{
+    if (B->staging_bytes >= bytes) return GW_OK;
+    if (B->d_staging) HIP_TRY(hipFree(B->d_staging));
+    B->d_staging = nullptr;
+    B->staging_bytes = 0;
+    HIP_TRY(hipMalloc(&B->d_staging, bytes));
+    B->staging_bytes = bytes;
+    return GW_OK;
+}
+
+// staging layout for one cell of n CRUs: CaSS[4n] CaJSR[n] JLCC[4n] (f64), LCC LCCi ClCh [4n] RyR[24n] (i32)
+size_t staging_size(int64_t n) { return static_cast<size_t>(n) * (9 * 8 + 36 * 4); }
+
+StateView staging_view(void *base, int64_t n)
+{
+    StateView v;
+    char *p = static_cast<char *>(base);
+    v.CaSS = reinterpret_cast<double *>(p); p += 4 * n * 8;
+    v.CaJSR = reinterpret_cast<double *>(p); p += n * 8;
+    v.JLCC = reinterpret_cast<double *>(p); p += 4 * n * 8;
+    v.LCC = reinterpret_cast<int32_t *>(p); p += 4 * n * 4;
+    v.LCCi = reinterpret_cast<int32_t *>(p); p += 4 * n * 4;
+    v.ClCh = reinterpret_cast<int32_t *>(p); p += 4 * n * 4;
+    v.RyR = reinterpret_cast<int32_t *>(p);
+    return v;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+void gw_params_default(gw_params_t *p)
+{
+    if (p) *p = default_params();
+}
+
+void gw_globals_default(gw_globals_t *g)
+{
+    if (g) *g = default_globals();
+}
+
+int gw_prng_from_name(const char *name)
+{
+    if (!name) return -1;
+    static const char *names[GW_PRNG_COUNT] = {"xoshiro256**", "xoshiro256++", "xoshiro256+",
+                                               "xoroshiro128**", "xoroshiro128++", "xoroshiro128+"};
+    for (int i = 0; i < GW_PRNG_COUNT; ++i)
+        if (std::strcmp(name, names[i]) == 0) return i;
+    return -1;
+}
+
+int gw_version(void) { return GW_CAPI_VERSION; }
+
+int gw_rest_state(int32_t n_cru, uint64_t seed, int32_t prng_id, gw_cru_state_t *out)
+{
+    if (n_cru <= 0 || !out || prng_id < 0 || prng_id >= GW_PRNG_COUNT) return GW_E_INVALID_ARG;
+    for (int64_t i = 0; i < n_cru; ++i) {
+        double ca[4], jsr;
+        uint32_t chan;
+        uint64_t ryr[4];
+        sample_rest_cru_any(prng_id, seed, i, ca, jsr, chan, ryr);
+        for (int j = 0; j < 4; ++j) {
+            if (out->CaSS) out->CaSS[i * 4 + j] = ca[j];
+            if (out->JLCC) out->JLCC[i * 4 + j] = 0.0;
+            if (out->LCC) out->LCC[i * 4 + j] = chan_lcc(chan, j);
+            if (out->LCC_inactivation) out->LCC_inactivation[i * 4 + j] = chan_y(chan, j);
+            if (out->ClCh) out->ClCh[i * 4 + j] = chan_cl(chan, j);
+            if (out->RyR)
+                for (int k = 0; k < 6; ++k) out->RyR[(i * 4 + j) * 6 + k] = ryr_get(ryr[j], k);
+        }
+        if (out->CaJSR) out->CaJSR[i] = jsr;
+    }
+    return GW_OK;
+}
+
+int gw_ctx_create(gw_ctx **out, int device, char *err, size_t errlen)
+{
+    if (!out) {
+        set_err(err, errlen, "out is null");
+        return GW_E_INVALID_ARG;
+    }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_err(err, errlen, "no HIP device is visible");
+        return GW_E_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        set_err(err, errlen, "device %d out of range (%d devices)", device, n);
+        return GW_E_INVALID_ARG;
+    }
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_err(err, errlen, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+        return GW_E_NO_DEVICE;
+    }
+    HIP_TRY(hipSetDevice(device));
+    gw_ctx *c = new gw_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_err(err, errlen, "hipStreamCreate failed");
+        return GW_E_HIP;
+    }
+    *out = c;
+    return GW_OK;
+}
+
+int gw_ctx_destroy(gw_ctx *ctx)
+{
+    if (!ctx) return GW_OK;
+    while (!ctx->batches.empty()) gw_batch_destroy(ctx->batches.back());
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GW_OK;
+}
+
+int gw_batch_destroy(gw_batch *B)
+{
+    if (!B) return GW_OK;
+    auto &live = B->ctx->batches;
+    live.erase(std::remove(live.begin(), live.end(), B), live.end());
+    (void)hipSetDevice(B->ctx->device);
+    (void)hipStreamSynchronize(B->ctx->stream);
+    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.chan, B->a.ryr, B->a.rng, B->a.dig, B->a.partials,
+                    B->a.counters, B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double *p : B->istim_dev)
+        if (p) (void)hipFree(p);
+    delete B;
+    return GW_OK;
+}
+
+int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, int32_t prng_id,
+                    double dt, const gw_globals_t *g0, gw_batch **out, char *err, size_t errlen)
+{
+    if (!ctx || !cells || !out || ncells <= 0) {
+        set_err(err, errlen, "gw_batch_create: null argument or ncells <= 0");
+        return GW_E_INVALID_ARG;
+    }
+    *out = nullptr;
+    if (!(dt > 0.0) || !std::isfinite(dt)) {
+        set_err(err, errlen, "step size must be > 0");
+        return GW_E_INVALID_ARG;
+    }
+    if (prng_id < 0 || prng_id >= GW_PRNG_COUNT) {
+        set_err(err, errlen, "unknown PRNG id %d", prng_id);
+        return GW_E_INVALID_ARG;
+    }
+    if (precision != GW_PREC_FP64) {
+        set_err(err, errlen, "precision %d is not available in this build", precision);
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    gw_batch *B = new gw_batch;
+    B->ctx = ctx;
+    B->ncells = ncells;
+    B->precision = precision;
+    B->prng = prng_id;
+    B->dt = dt;
+    B->cells_h.resize(ncells);
+    B->istim_dev.assign(ncells, nullptr);
+    int64_t slot = 0, blk = 0;
+    for (int c = 0; c < ncells; ++c) {
+        const gw_cell_desc_t &d = cells[c];
+        if (d.n_cru <= 0 || d.n_cru > (1 << 20)) {
+            set_err(err, errlen, "cell %d: n_cru must be in 1..%d", c, 1 << 20);
+            delete B;
+            return GW_E_INVALID_ARG;
+        }
+        int rc = validate_params(d.params, err, errlen);
+        if (rc != GW_OK) {
+            delete B;
+            return rc;
+        }
+        if (d.protocol.kind < GW_PROTO_TABLE || d.protocol.kind > GW_PROTO_CLAMP) {
+            set_err(err, errlen, "cell %d: unknown protocol kind %d", c, d.protocol.kind);
+            delete B;
+            return GW_E_INVALID_ARG;
+        }
+        DevCell &C = B->cells_h[c];
+        std::memset(&C, 0, sizeof(C));
+        C.p = d.params;
+        cell_consts_init(C.c, C.p, d.n_cru);
+        ssa_params_init(C.sp, C.p, C.c);
+        C.proto = d.protocol;
+        C.seed = d.seed;
+        C.n_cru = d.n_cru;
+        C.nblk = (d.n_cru + kBlock - 1) / kBlock;
+        C.slot0 = slot;
+        C.blk0 = blk;
+        slot += static_cast<int64_t>(C.nblk) * kBlock;
+        blk += C.nblk;
+        for (int b = 0; b < C.nblk; ++b) B->block_cell_h.push_back(c);
+    }
+    B->ntot = slot;
+    B->nblocks = blk;
+    const int64_t N = B->ntot;
+    KArgs &a = B->a;
+    int rc = GW_OK;
+    if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
+        (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.chan, N, err, errlen)) ||
+        (rc = dmalloc(&a.ryr, 4 * N, err, errlen)) || (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
+        (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, B->nblocks, err, errlen)) ||
+        (rc = dmalloc(&a.counters, ncells, err, errlen)) || (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
+        (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
+        (rc = dmalloc(&B->d_block_cell, B->nblocks, err, errlen))) {
+        gw_batch_destroy(B);
+        return rc;
+    }
+    a.cells = B->d_cells;
+    a.dyn = B->d_dyn;
+    a.block_cell = B->d_block_cell;
+    a.ntot = N;
+    a.dt = dt;
+    a.ncells = ncells;
+    a.frozen = 0;
+    a.trace_every = 0;
+    a.trace_cap = 0;
+    a.trace = nullptr;
+    a.prng = prng_id;
+    // per-cell stimulus tables are attached later; globals start at g0 or the defaults
+    std::vector<CellDyn> dyn(ncells);
+    B->g0.resize(ncells);
+    for (int c = 0; c < ncells; ++c) {
+        std::memset(&dyn[c], 0, sizeof(CellDyn));
+        dyn[c].g = g0 ? *g0 : default_globals();
+        dyn[c].clamp = (cells[c].protocol.kind == GW_PROTO_CLAMP) ? 1 : 0;
+        B->g0[c] = dyn[c].g;
+    }
+    auto fail = [&](hipError_t e) {
+        set_err(err, errlen, "HIP error during batch creation: %s", hipGetErrorString(e));
+        gw_batch_destroy(B);
+        return GW_E_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMemcpyAsync(B->d_cells, B->cells_h.data(), ncells * sizeof(DevCell), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(B->d_dyn, dyn.data(), ncells * sizeof(CellDyn), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(B->d_block_cell, B->block_cell_h.data(), B->nblocks * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemsetAsync(a.counters, 0, ncells * sizeof(uint32_t), ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemsetAsync(a.iters_total, 0, sizeof(unsigned long long), ctx->stream)) != hipSuccess) return fail(e);
+    hipLaunchKernelGGL(k_seed, dim3(B->nblocks), dim3(kBlock), 0, ctx->stream, a);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((rc = sample_rest(B, err, errlen)) != GW_OK) {
+        gw_batch_destroy(B);
+        return rc;
+    }
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
+    ctx->batches.push_back(B);
+    *out = B;
+    return GW_OK;
+}
+
+int gw_batch_set_cru_state(gw_batch *B, int32_t cell, const gw_cru_state_t *c, char *err, size_t errlen)
+{
+    if (!B || !c || cell < 0 || cell >= B->ncells || !c->CaSS || !c->CaJSR || !c->LCC || !c->LCC_inactivation ||
+        !c->RyR || !c->ClCh) {
+        set_err(err, errlen, "gw_batch_set_cru_state: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    const int64_t n = B->cells_h[cell].n_cru;
+    for (int64_t i = 0; i < 4 * n; ++i) {
+        if (c->LCC[i] < 1 || c->LCC[i] > 12) {
+            set_err(err, errlen, "LCC states must be integers in 1..12 (got %d at flat index %lld)", c->LCC[i], static_cast<long long>(i));
+            return GW_E_INVALID_ARG;
+        }
+        if ((c->LCC_inactivation[i] & ~1) || (c->ClCh[i] & ~1)) {
+            set_err(err, errlen, "LCC_inactivation and ClCh must be 0 or 1 (flat index %lld)", static_cast<long long>(i));
+            return GW_E_INVALID_ARG;
+        }
+        if (!std::isfinite(c->CaSS[i])) {
+            set_err(err, errlen, "CaSS must be finite");
+            return GW_E_INVALID_ARG;
+        }
+    }
+    for (int64_t i = 0; i < 24 * n; ++i)
+        if (c->RyR[i] < -100 || c->RyR[i] > 100) {
+            set_err(err, errlen, "RyR counts must lie in -100..100");
+            return GW_E_INVALID_ARG;
+        }
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(c->CaJSR[i])) {
+            set_err(err, errlen, "CaJSR must be finite");
+            return GW_E_INVALID_ARG;
+        }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = ensure_staging(B, staging_size(n), err, errlen);
+    if (rc) return rc;
+    StateView v = staging_view(B->d_staging, n);
+    hipStream_t s = B->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(v.CaSS, c->CaSS, 4 * n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.CaJSR, c->CaJSR, n * 8, hipMemcpyHostToDevice, s));
+    if (c->JLCC) HIP_TRY(hipMemcpyAsync(v.JLCC, c->JLCC, 4 * n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.LCC, c->LCC, 4 * n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.LCCi, c->LCC_inactivation, 4 * n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.ClCh, c->ClCh, 4 * n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.RyR, c->RyR, 24 * n * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_put_state, dim3((n + 255) / 256), dim3(256), 0, s, B->a, cell, v, c->JLCC ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    B->prepared = false;
+    return GW_OK;
+}
+
+int gw_batch_get_cru_state(gw_batch *B, int32_t cell, gw_cru_state_t *c, char *err, size_t errlen)
+{
+    if (!B || !c || cell < 0 || cell >= B->ncells) {
+        set_err(err, errlen, "gw_batch_get_cru_state: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    const int64_t n = B->cells_h[cell].n_cru;
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = ensure_staging(B, staging_size(n), err, errlen);
+    if (rc) return rc;
+    StateView v = staging_view(B->d_staging, n);
+    hipStream_t s = B->ctx->stream;
+    hipLaunchKernelGGL(k_get_state, dim3((n + 255) / 256), dim3(256), 0, s, B->a, cell, v);
+    HIP_TRY(hipGetLastError());
+    if (c->CaSS) HIP_TRY(hipMemcpyAsync(c->CaSS, v.CaSS, 4 * n * 8, hipMemcpyDeviceToHost, s));
+    if (c->CaJSR) HIP_TRY(hipMemcpyAsync(c->CaJSR, v.CaJSR, n * 8, hipMemcpyDeviceToHost, s));
+    if (c->JLCC) HIP_TRY(hipMemcpyAsync(c->JLCC, v.JLCC, 4 * n * 8, hipMemcpyDeviceToHost, s));
+    if (c->LCC) HIP_TRY(hipMemcpyAsync(c->LCC, v.LCC, 4 * n * 4, hipMemcpyDeviceToHost, s));
+    if (c->LCC_inactivation) HIP_TRY(hipMemcpyAsync(c->LCC_inactivation, v.LCCi, 4 * n * 4, hipMemcpyDeviceToHost, s));
+    if (c->ClCh) HIP_TRY(hipMemcpyAsync(c->ClCh, v.ClCh, 4 * n * 4, hipMemcpyDeviceToHost, s));
+    if (c->RyR) HIP_TRY(hipMemcpyAsync(c->RyR, v.RyR, 24 * n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GW_OK;
+}
+
+int gw_batch_set_globals(gw_batch *B, int32_t cell, const gw_globals_t *g, char *err, size_t errlen)
+{
+    if (!B || !g || cell < 0 || cell >= B->ncells) {
+        set_err(err, errlen, "gw_batch_set_globals: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    HIP_TRY(hipMemcpyAsync(&B->d_dyn[cell].g, g, sizeof(gw_globals_t), hipMemcpyHostToDevice, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    B->prepared = false;
+    return GW_OK;
+}
+
+int gw_batch_get_globals(gw_batch *B, gw_globals_t *g_out, char *err, size_t errlen)
+{
+    if (!B || !g_out) {
+        set_err(err, errlen, "gw_batch_get_globals: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    for (int c = 0; c < B->ncells; ++c)
+        HIP_TRY(hipMemcpyAsync(&g_out[c], &B->d_dyn[c].g, sizeof(gw_globals_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+int gw_batch_set_stimulus_table(gw_batch *B, int32_t cell, const double *istim, int64_t n, char *err, size_t errlen)
+{
+    if (!B || cell < 0 || cell >= B->ncells || (n > 0 && !istim) || n < 0) {
+        set_err(err, errlen, "gw_batch_set_stimulus_table: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    if (B->istim_dev[cell]) HIP_TRY(hipFree(B->istim_dev[cell]));
+    B->istim_dev[cell] = nullptr;
+    double *d = nullptr;
+    if (n > 0) {
+        HIP_TRY(hipMalloc(&d, n * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(d, istim, n * sizeof(double), hipMemcpyHostToDevice, B->ctx->stream));
+    }
+    B->istim_dev[cell] = d;
+    B->cells_h[cell].istim = d;
+    B->cells_h[cell].istim_len = n;
+    B->cells_h[cell].proto.kind = GW_PROTO_TABLE;
+    HIP_TRY(hipMemcpyAsync(&B->d_cells[cell], &B->cells_h[cell], sizeof(DevCell), hipMemcpyHostToDevice, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    B->prepared = false;
+    return GW_OK;
+}
+
+int gw_batch_step(gw_batch *B, int64_t nsteps, char *err, size_t errlen)
+{
+    if (!B || nsteps < 0) {
+        set_err(err, errlen, "gw_batch_step: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = launch_cfg_check(B, err, errlen);
+    if (rc) return rc;
+    rc = prepare(B, err, errlen);
+    if (rc) return rc;
+    if (B->frozen) {
+        // one launch per chunk; a chunk is at most ~2^26 CRU-steps of work
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(1 << 18, (1LL << 26) / std::max<int64_t>(1, B->ntot)));
+        for (int64_t done = 0; done < nsteps;) {
+            const int64_t n = std::min(chunk, nsteps - done);
+            launch_frozen_steps(B, B->steps_done, n);
+            HIP_TRY(hipGetLastError());
+            done += n;
+            B->steps_done += n;
+            rc = check_status(B, err, errlen);
+            if (rc) return rc;
+        }
+        return GW_OK;
+    }
+    const int64_t chunk = 4096;
+    for (int64_t done = 0; done < nsteps;) {
+        const int64_t n = std::min(chunk, nsteps - done);
+        launch_steps(B, n);
+        HIP_TRY(hipGetLastError());
+        done += n;
+        B->steps_done += n;
+        rc = check_status(B, err, errlen);
+        if (rc) return rc;
+    }
+    return GW_OK;
+}
+
+int gw_batch_get_time(gw_batch *B, double *t, int64_t *steps_done)
+{
+    if (!B) return GW_E_INVALID_ARG;
+    (void)hipSetDevice(B->ctx->device);
+    if (!B->prepared) {
+        if (t) *t = 0.0;
+        if (steps_done) *steps_done = 0;
+        return GW_OK;
+    }
+    double tt = 0.0;
+    int64_t st = 0;
+    (void)hipMemcpyAsync(&tt, &B->d_dyn[0].t, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream);
+    (void)hipMemcpyAsync(&st, &B->d_dyn[0].step, sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream);
+    if (hipStreamSynchronize(B->ctx->stream) != hipSuccess) return GW_E_HIP;
+    if (t) *t = tt;
+    if (steps_done) *steps_done = st;
+    return GW_OK;
+}
+
+int gw_batch_get_obs(gw_batch *B, gw_cell_obs_t *obs, char *err, size_t errlen)
+{
+    if (!B || !obs) {
+        set_err(err, errlen, "gw_batch_get_obs: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    for (int c = 0; c < B->ncells; ++c)
+        HIP_TRY(hipMemcpyAsync(&obs[c], &B->d_dyn[c].obs, sizeof(gw_cell_obs_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+int gw_batch_reset_obs(gw_batch *B, char *err, size_t errlen)
+{
+    if (!B) return GW_E_INVALID_ARG;
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = prepare(B, err, errlen);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reset_obs, dim3((B->ncells + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, B->ncells);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+int gw_batch_enable_trace(gw_batch *B, int32_t every, int64_t capacity, char *err, size_t errlen)
+{
+    if (!B || every < 0 || capacity < 0) {
+        set_err(err, errlen, "gw_batch_enable_trace: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    if (B->a.trace) HIP_TRY(hipFree(B->a.trace));
+    B->a.trace = nullptr;
+    B->a.trace_every = 0;
+    B->a.trace_cap = 0;
+    if (every > 0 && capacity > 0) {
+        HIP_TRY(hipMalloc(&B->a.trace, static_cast<size_t>(capacity) * B->ncells * 4 * sizeof(double)));
+        B->a.trace_every = every;
+        B->a.trace_cap = capacity;
+    }
+    for (int c = 0; c < B->ncells; ++c) HIP_TRY(hipMemsetAsync(&B->d_dyn[c].trace_rows, 0, sizeof(int64_t), B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+int gw_batch_get_trace(gw_batch *B, double *out, int64_t max_rows, int64_t *rows, char *err, size_t errlen)
+{
+    if (!B || !rows) {
+        set_err(err, errlen, "gw_batch_get_trace: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int64_t r = 0;
+    HIP_TRY(hipMemcpyAsync(&r, &B->d_dyn[0].trace_rows, sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    r = std::min(r, max_rows);
+    if (r > 0 && out)
+        HIP_TRY(hipMemcpyAsync(out, B->a.trace, static_cast<size_t>(r) * B->ncells * 4 * sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
+    for (int c = 0; c < B->ncells; ++c) HIP_TRY(hipMemsetAsync(&B->d_dyn[c].trace_rows, 0, sizeof(int64_t), B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    *rows = r;
+    return GW_OK;
+}
+
+int gw_batch_get_event_digest(gw_batch *B, int32_t cell, uint64_t *count, uint64_t *hash, char *err, size_t errlen)
+{
+    if (!B || cell < 0 || cell >= B->ncells || !count || !hash) {
+        set_err(err, errlen, "gw_batch_get_event_digest: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    if (!B->a.dig) {
+        set_err(err, errlen, "event digests are not kept by this batch");
+        return GW_E_INVALID_ARG;
+    }
+    const DevCell &C = B->cells_h[cell];
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    HIP_TRY(hipMemcpyAsync(count, B->a.dig + C.slot0, C.n_cru * sizeof(uint64_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hash, B->a.dig + B->ntot + C.slot0, C.n_cru * sizeof(uint64_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+int gw_batch_get_iterations(gw_batch *B, uint64_t *iters)
+{
+    if (!B || !iters) return GW_E_INVALID_ARG;
+    (void)hipSetDevice(B->ctx->device);
+    unsigned long long v = 0;
+    (void)hipMemcpyAsync(&v, B->a.iters_total, sizeof(v), hipMemcpyDeviceToHost, B->ctx->stream);
+    if (hipStreamSynchronize(B->ctx->stream) != hipSuccess) return GW_E_HIP;
+    *iters = v;
+    return GW_OK;
+}
+
+int gw_batch_set_launch(gw_batch *B, int32_t crus_per_thread, int32_t block, char *err, size_t errlen)
+{
+    if (!B) return GW_E_INVALID_ARG;
+    if ((crus_per_thread != 0 && crus_per_thread != 1) || (block != 0 && block != kBlock)) {
+        set_err(err, errlen, "launch shape %d CRUs/thread x %d threads is not available in this build", crus_per_thread, block);
+        return GW_E_INVALID_ARG;
+    }
+    B->crus_per_thread = 1;
+    B->block = kBlock;
+    return GW_OK;
+}
+
+int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, int64_t *launches, char *err, size_t errlen)
+{
+    if (!B || nsteps <= 0 || !ms) {
+        set_err(err, errlen, "gw_batch_time_steps: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = prepare(B, err, errlen);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, B->ctx->stream));
+    launch_steps(B, nsteps);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, B->ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = f;
+    if (launches) *launches = nsteps;
+    return check_status(B, err, errlen);
+}
+
+// Internal switches used by gw_run and the replay tests.
+int gw_batch_set_mode(gw_batch *B, int32_t frozen, int32_t digest)
+{
+    if (!B) return GW_E_INVALID_ARG;
+    B->frozen = frozen != 0;
+    B->a.frozen = frozen != 0;
+    B->digest = digest != 0;
+    return GW_OK;
+}
+
+int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
+           int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
+           int32_t precision, gw_record_t *out, char *err, size_t errlen)
+{
+    if (!ctx || !p || n_cru <= 0 || num_steps < 0 || record_every <= 0) {
+        set_err(err, errlen, "gw_run: invalid arguments (n_cru > 0, num_steps >= 0, record_every > 0 required)");
+        return GW_E_INVALID_ARG;
+    }
+    gw_cell_desc_t d;
+    std::memset(&d, 0, sizeof(d));
+    d.n_cru = n_cru;
+    d.seed = seed;
+    d.params = *p;
+    d.protocol.kind = GW_PROTO_TABLE;
+    gw_batch *B = nullptr;
+    int rc = gw_batch_create(ctx, 1, &d, precision, prng_id, dt, g0, &B, err, errlen);
+    if (rc) return rc;
+    gw_batch_set_mode(B, 0, 0);
+    if (c0) {
+        gw_cru_state_t c = *c0;
+        c.JLCC = nullptr;  // a reference run starts with JLCC = 0 (GW.hpp:104-113)
+        rc = gw_batch_set_cru_state(B, 0, &c, err, errlen);
+        if (rc) { gw_batch_destroy(B); return rc; }
+    }
+    if (num_steps > 0 && istim) {
+        rc = gw_batch_set_stimulus_table(B, 0, istim, num_steps, err, errlen);
+        if (rc) { gw_batch_destroy(B); return rc; }
+    }
+    const int64_t n = n_cru;
+    double t = 0.0;
+    int64_t counter = 0;
+    std::vector<double> dummyd;
+    int64_t i = 0;
+    while (i < num_steps) {
+        // next record after step i_rec (i_rec % record_every == 0, i_rec >= i)
+        const int64_t i_rec = ((i + record_every - 1) / record_every) * record_every;
+        const int64_t upto = std::min(num_steps, i_rec + 1);
+        rc = gw_batch_step(B, upto - i, err, errlen);
+        if (rc) { gw_batch_destroy(B); return rc; }
+        for (int64_t k = i; k < upto; ++k) t += dt;
+        i = upto;
+        if (out && i == i_rec + 1) {
+            if (counter < out->num_records) {
+                gw_globals_t g;
+                rc = gw_batch_get_globals(B, &g, err, errlen);
+                if (rc) { gw_batch_destroy(B); return rc; }
+                const int64_t k = counter;
+#define PUT(f, v) if (out->f) out->f[k] = (v)
+                PUT(t, t); PUT(V, g.V); PUT(m, g.m); PUT(h, g.h); PUT(j, g.j); PUT(Nai, g.Nai); PUT(Ki, g.Ki);
+                PUT(Cai, g.Cai); PUT(CaNSR, g.CaNSR); PUT(CaLTRPN, g.CaLTRPN); PUT(CaHTRPN, g.CaHTRPN); PUT(xKs, g.xKs);
+#undef PUT
+                if (out->XKr) for (int q = 0; q < 5; ++q) out->XKr[k * 5 + q] = g.Kr[q];
+                if (out->XKv14) for (int q = 0; q < 10; ++q) out->XKv14[k * 10 + q] = g.Kv14[q];
+                if (out->XKv43) for (int q = 0; q < 10; ++q) out->XKv43[k * 10 + q] = g.Kv43[q];
+                gw_cru_state_t c;
+                c.CaSS = out->CaSS ? out->CaSS + k * n * 4 : nullptr;
+                c.CaJSR = out->CaJSR ? out->CaJSR + k * n : nullptr;
+                c.LCC = out->LCC ? out->LCC + k * n * 4 : nullptr;
+                c.LCC_inactivation = out->LCC_inactivation ? out->LCC_inactivation + k * n * 4 : nullptr;
+                c.ClCh = out->ClCh ? out->ClCh + k * n * 4 : nullptr;
+                c.RyR = out->RyR ? out->RyR + k * n * 24 : nullptr;
+                c.JLCC = nullptr;
+                rc = gw_batch_get_cru_state(B, 0, &c, err, errlen);
+                if (rc) { gw_batch_destroy(B); return rc; }
+            }
+            ++counter;
+        }
+    }
+    gw_batch_destroy(B);
+    return GW_OK;
+}
+
+int gw_run_batch(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, double dt, int64_t nsteps,
+                 gw_globals_t *g_out, gw_cell_obs_t *obs_out, char *err, size_t errlen)
+{
+    gw_batch *B = nullptr;
+    int rc = gw_batch_create(ctx, ncells, cells, precision, GW_PRNG_XOSHIRO256SS, dt, nullptr, &B, err, errlen);
+    if (rc) return rc;
+    gw_batch_set_mode(B, 0, 0);
+    rc = gw_batch_step(B, nsteps, err, errlen);
+    if (!rc && g_out) rc = gw_batch_get_globals(B, g_out, err, errlen);
+    if (!rc && obs_out) rc = gw_batch_get_obs(B, obs_out, err, errlen);
+    gw_batch_destroy(B);
+    return rc;
+}
+
+}  // extern "C"

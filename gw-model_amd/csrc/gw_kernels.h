@@ -1,0 +1,74 @@
+// gw_kernels.h -- device data layout shared by the kernels and the host side.
+#pragma once
+
+#include "gw_ssa.h"
+
+namespace gw {
+
+constexpr int kBlock = 256;        // threads per SSA block (one CRU per thread)
+constexpr int kWave = 64;
+
+// Static description of one cell on the device.
+struct DevCell {
+    gw_params_t p;
+    CellConsts c;
+    SsaParams sp;
+    gw_protocol_t proto;
+    const double *istim;   // per-step stimulus table (GW_PROTO_TABLE) or null
+    int64_t istim_len;
+    int64_t slot0;         // first CRU slot of this cell in the SoA arrays
+    int64_t blk0;          // first SSA block of this cell
+    uint64_t seed;
+    int32_t n_cru;
+    int32_t nblk;
+};
+
+// Dynamic per-cell state, advanced by the last block of each step.
+struct CellDyn {
+    gw_globals_t g;      // globals at the start of the current step
+    gw_globals_t dg;     // increments formed before the current step's SSA
+    StepConsts k;        // SSA constants of the current step
+    double t;            // accumulated time at the start of the current step
+    double Istim;        // stimulus of the current step
+    double sums[6];      // JLCC, Jxfer, Jtr, ClCh, open RyR, Jrel of the last SSA
+    int64_t step;        // index of the current step
+    int64_t trace_rows;
+    int32_t status;
+    int32_t clamp;
+    gw_cell_obs_t obs;
+};
+
+// Block partial of the per-step reductions.
+struct Partial {
+    Fx jlcc, jxfer, jtr, jrel;
+    int64_t clch, open;
+    uint64_t iters;
+    int32_t err;
+    int32_t _pad;
+};
+
+struct KArgs {
+    const DevCell *cells;
+    CellDyn *dyn;
+    const int32_t *block_cell;
+    double *CaSS;      // [4][ntot]
+    double *CaJSR;     // [ntot]
+    double *JLCC;      // [4][ntot]
+    uint32_t *chan;    // [ntot]
+    uint64_t *ryr;     // [4][ntot]
+    uint64_t *rng;     // [4][ntot]
+    uint64_t *dig;     // [2][ntot] (count, hash) when digests are kept
+    Partial *partials; // [nblocks]
+    uint32_t *counters;// [ncells]
+    double *trace;     // [cap][ncells][4]
+    unsigned long long *iters_total;
+    int64_t ntot;
+    int64_t trace_cap;
+    double dt;
+    int32_t ncells;
+    int32_t frozen;
+    int32_t trace_every;
+    int32_t prng;
+};
+
+}  // namespace gw

@@ -1,0 +1,379 @@
+// gw_model.h -- the Greenstein-Winslow global model on the device (and host):
+// derived constants, the per-step constant tables the SSA reads, and the
+// explicit-Euler global ODE step. Every expression keeps the reference's
+// operation order so that results are bit-identical to the CPU oracle.
+#pragma once
+
+#include "gw_math.h"
+#include "../../include/gw_capi.h"
+
+namespace gw {
+
+constexpr double kFaraday = 96.5;   // src/includes/common.hpp:9
+constexpr double kGasConst = 8.314; // src/includes/common.hpp:10
+
+GW_HD double sq(double x) { return x * x; }
+GW_HD double cube(double x) { return x * x * x; }
+
+// Derived constants of GW::Constants (src/includes/GW_utils.tpp:67-116).
+struct CellConsts {
+    double RT_F, F_RT, CRU_factor, CSA_FVcyto, VSS_Vcyto, Vcyto_VNSR, VJSR_VNSR;
+    double gamma0a, gamma0a2, gamma0a3, gamma0a4, binv, omega_b, omega_b2, omega_b3, omega_b4;
+    double BSR_const, BSL_const, VSS_VJSR, CSQN_const, JLCC_const, Cao_scaled;
+    double Nao3, INaCa_const, sigma, INaK_const, sqrtKo, PKv14_Csc, Ito2_const, IK1_const, ICaL_const, CMDN_const;
+};
+
+GW_HD void cell_consts_init(CellConsts &c, const gw_params_t &p, int nCRU)
+{
+    c.RT_F = kGasConst * p.T / kFaraday;
+    c.F_RT = 1.0 / c.RT_F;
+    c.CRU_factor = static_cast<double>(p.NCaRU) / static_cast<double>(nCRU);
+    c.CSA_FVcyto = p.CSA / (1000 * p.Vcyto * kFaraday);
+    c.VSS_Vcyto = p.VSS / p.Vcyto;
+    c.Vcyto_VNSR = p.Vcyto / p.VNSR;
+    c.VJSR_VNSR = p.VJSR / p.VNSR;
+    c.gamma0a = p.gamma0 * p.a;
+    c.gamma0a2 = c.gamma0a * p.a;
+    c.gamma0a3 = c.gamma0a2 * p.a;
+    c.gamma0a4 = c.gamma0a3 * p.a;
+    c.binv = 1.0 / p.b;
+    c.omega_b = p.omega * c.binv;
+    c.omega_b2 = c.omega_b * c.binv;
+    c.omega_b3 = c.omega_b2 * c.binv;
+    c.omega_b4 = c.omega_b3 * c.binv;
+    c.BSR_const = p.KBSR * p.BSRT;
+    c.BSL_const = p.KBSL * p.BSLT;
+    c.VSS_VJSR = p.VSS / p.VJSR;
+    c.CSQN_const = p.KCSQN * p.CSQNT;
+    c.JLCC_const = 2.0e6 * p.PCaL / p.VSS;
+    c.Cao_scaled = 0.341 * p.Cao;
+    c.Nao3 = p.Nao * p.Nao * p.Nao;
+    c.INaCa_const = 5000.0 * p.kNaCa / ((p.KmNa * p.KmNa * p.KmNa + p.Nao * p.Nao * p.Nao) * (p.KmCa + p.Cao));
+    c.sigma = (gw::exp(p.Nao / 67.3) - 1.0) / 7.0;
+    c.INaK_const = p.INaKmax * p.Ko / (p.Ko + p.KmKo);
+    c.sqrtKo = __builtin_sqrt(p.Ko);
+    c.PKv14_Csc = p.PKv14 / p.Csc;
+    c.Ito2_const = 1.0e9 * p.Pto2 * kFaraday * c.CRU_factor / p.CSA;
+    c.IK1_const = p.Ko / (p.Ko + p.KmK1);
+    c.ICaL_const = -1000.0 * (2.0 * kFaraday * p.VSS) * c.CRU_factor / p.CSA;
+    c.CMDN_const = p.KCMDN * p.CMDNT;
+}
+
+// Constants the SSA reads during one step (frozen inside GW_model::SSA,
+// src/includes/GW.tpp:214-233). The LCC propensities of a subunit in state s
+// are tabulated so that (r0 + r1) + r2 = lccP[s] + lccQ[s] * CaSS bit for bit;
+// lccR holds r0, r1, r2 with the CaSS-proportional slot set to zero.
+struct StepConsts {
+    double Cai, CaNSR;
+    double inact0, inact1;   // (y == 0) ? yinf/tau : (1 - yinf)/tau  (GW_utils.hpp:381)
+    double JLCC_mult, JLCC_exp;
+    double lccP[13], lccQ[13];
+    double lccR0[13], lccR1[13], lccR2[13];
+    uint32_t step_lo;
+    uint32_t _pad;
+};
+
+// GW::alphaLCC .. tauLCC (GW_utils.hpp:365-375)
+GW_HD double alphaLCC(double V) { return 2.0 * gw::exp(0.012 * (V - 35.0)); }
+GW_HD double betaLCC(double V) { return 0.0882 * gw::exp(-0.05 * (V - 35.0)); }
+GW_HD double yinfLCC(double V) { return 0.4 / (1.0 + gw::exp((V + 12.5) / 5.0)) + 0.6; }
+GW_HD double tauLCC(double V) { return 340.0 / (1.0 + gw::exp((V + 30.0) / 12.0)) + 60.0; }
+
+// Per-step constants of GW_model::SSA (GW.tpp:216-221) and the LCC rate table
+// of update_LCC_rates (GW_utils.tpp:118-186). At V == 0 the multiplier takes
+// its analytic limit JLCC_const / 2 instead of 0/0.
+GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, double expmVF_RT,
+                       double Cai, double CaNSR, uint32_t step_lo)
+{
+    const double al = alphaLCC(V);
+    const double be = betaLCC(V);
+    const double yinf = yinfLCC(V);
+    const double tau = tauLCC(V);
+    k.Cai = Cai;
+    k.CaNSR = CaNSR;
+    k.inact0 = yinf / tau;
+    k.inact1 = (1.0 - yinf) / tau;
+    k.JLCC_exp = sq(1.0 / expmVF_RT);
+    k.JLCC_mult = (V == 0.0) ? c.JLCC_const / 2.0 : c.JLCC_const * V * c.F_RT / (k.JLCC_exp - 1);
+    k.step_lo = step_lo;
+    k._pad = 0;
+    // r0, r1, r2 of each state; the CaSS-proportional rate carries its coefficient in Q.
+    double r0[13], r1[13], r2[13], q[13];
+    for (int s = 0; s < 13; ++s) { r0[s] = 0; r1[s] = 0; r2[s] = 0; q[s] = 0; }
+    r0[1] = 4 * al;           q[1] = p.gamma0;          // r1 = gamma0 CaSS
+    r0[2] = be;               r1[2] = 3 * al;           q[2] = c.gamma0a;   // r2 = gamma0 a CaSS
+    r0[3] = 2 * be;           r1[3] = 2 * al;           q[3] = c.gamma0a2;
+    r0[4] = 3 * be;           r1[4] = al;               q[4] = c.gamma0a3;
+    r0[5] = 4 * be;           r1[5] = p.f;              q[5] = c.gamma0a4;
+    r0[6] = p.g;
+    r0[7] = p.omega;          r1[7] = 4 * al * p.a;
+    r0[8] = c.omega_b;        r1[8] = be * c.binv;      r2[8] = 3 * al * p.a;
+    r0[9] = c.omega_b2;       r1[9] = 2 * be * c.binv;  r2[9] = 2 * al * p.a;
+    r0[10] = c.omega_b3;      r1[10] = 3 * be * c.binv; r2[10] = al * p.a;
+    r0[11] = c.omega_b4;      r1[11] = 4 * be * c.binv; r2[11] = p.f1;
+    r0[12] = p.g1;
+    for (int s = 0; s < 13; ++s) {
+        k.lccR0[s] = r0[s];
+        k.lccR1[s] = r1[s];
+        k.lccR2[s] = r2[s];
+        k.lccQ[s] = q[s];
+        // state 1 carries its CaSS term in r1: (r0 + g0 CaSS) + 0 == P + Q CaSS with P = r0.
+        // states 2..5 carry it in r2: (r0 + r1) + Q CaSS. Others: (r0 + r1) + r2 + 0.
+        if (s == 1) k.lccP[s] = r0[s];
+        else if (s >= 2 && s <= 5) k.lccP[s] = r0[s] + r1[s];
+        else k.lccP[s] = r0[s] + r1[s] + r2[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global ODE: update_QKr / update_QKv (GW.tpp:25-44, 64-123) and the
+// derivatives of update_V_and_concentration_derivatives, update_gate_derivatives,
+// update_Kr/Kv_derivatives (GW.tpp:126-212). dg receives dt * f(state) as the
+// reference forms it. Returns GW_OK or GW_E_NONPOSITIVE_CONC.
+// ---------------------------------------------------------------------------
+struct FluxSums {
+    double JLCC, Jxfer, Jtr, ClCh;
+};
+
+GW_HD double nernst(double Xi, double Xo, double RT_F, double z, bool &ok)
+{
+    if (Xi <= 0.0 || Xo <= 0.0) {   // the reference exits here (common.hpp:72-75)
+        ok = false;
+        return 0.0;
+    }
+    return gw::log(Xo / Xi) * RT_F / z;
+}
+
+GW_HD double alpha_m(double V) { return (V == -47.13) ? 3.2 : 0.32 * (V + 47.13) / (1.0 - gw::exp(-0.1 * (V + 47.13))); }
+GW_HD double beta_m(double V) { return 0.08 * gw::exp(-V / 11.0); }
+GW_HD double alpha_h(double V) { return (V >= -40.0) ? 0.0 : 0.135 * gw::exp(-(V + 80.0) / 6.8); }
+GW_HD double beta_h(double V)
+{
+    return (V >= -40.0) ? 1.0 / (0.13 * (1.0 + gw::exp(-(V + 10.66) / 11.1)))
+                        : 3.56 * gw::exp(0.079 * V) + 3.1e5 * gw::exp(0.35 * V);
+}
+GW_HD double alpha_j(double V)
+{
+    return (V >= -40.0) ? 0.0
+                        : (-127140.0 * gw::exp(0.2444 * V) - 3.474e-5 * gw::exp(-0.04391 * V)) * (V + 37.78) /
+                              (1.0 + gw::exp(0.311 * (V + 79.23)));
+}
+GW_HD double beta_j(double V)
+{
+    return (V >= -40.0) ? 0.3 * gw::exp(-2.535e-7 * V) / (1.0 + gw::exp(-0.1 * (V + 32.0)))
+                        : 0.1212 * gw::exp(-0.01052 * V) / (1.0 + gw::exp(-0.1378 * (V + 40.14)));
+}
+GW_HD double XKsinf(double V) { return 1.0 / (1.0 + gw::exp(-(V - 24.7) / 13.6)); }
+GW_HD double tauXKs_inv(double V)
+{
+    if (V == 10.0) return 0.0000719 / 0.148 + 0.000131 / 0.0687;  // analytic limit of 0/0
+    return 0.0000719 * (V - 10.0) / (1.0 - gw::exp(-0.148 * (V - 10.0))) +
+           0.000131 * (V - 10.0) / (gw::exp(0.0687 * (V - 10.0)) - 1.0);
+}
+
+struct QMatrices {
+    double Kr[25], Kv14[100], Kv43[100];
+};
+
+GW_HD void q_matrices(QMatrices &Q, const gw_params_t &p, double V)
+{
+    double *Kr = Q.Kr;
+#define KR(i, j) Kr[(i) * 5 + (j)]
+    KR(1, 2) = p.Kf;
+    KR(2, 1) = p.Kb;
+    KR(0, 1) = 0.0069 * gw::exp(0.0272 * V);
+    KR(0, 0) = -KR(0, 1);
+    KR(1, 0) = 0.0227 * gw::exp(-0.0431 * V);
+    KR(1, 1) = -(KR(1, 0) + KR(1, 2));
+    KR(2, 3) = 0.0218 * gw::exp(0.0262 * V);
+    KR(2, 4) = 1.29e-5 * gw::exp(2.71e-6 * V);
+    KR(2, 2) = -(KR(2, 1) + KR(2, 3) + KR(2, 4));
+    KR(3, 2) = 0.0009 * gw::exp(-0.0269 * V);
+    KR(3, 4) = 0.0622 * gw::exp(0.0120 * V);
+    KR(3, 3) = -(KR(3, 2) + KR(3, 4));
+    KR(4, 3) = 0.0059 * gw::exp(-0.0443 * V);
+    KR(4, 2) = KR(3, 2) * KR(4, 3) * KR(2, 4) / (KR(2, 3) * KR(3, 4));
+    KR(4, 4) = -(KR(4, 2) + KR(4, 3));
+#undef KR
+    const double aa14 = p.alphaa0Kv14 * gw::exp(p.aaKv14 * V);
+    const double aa43 = p.alphaa0Kv43 * gw::exp(p.aaKv43 * V);
+    const double ai14 = p.alphai0Kv14 * gw::exp(-p.aiKv14 * V);
+    const double ai43 = p.alphai0Kv43 * gw::exp(-p.aiKv43 * V);
+    const double ba14 = p.betaa0Kv14 * gw::exp(-p.baKv14 * V);
+    const double ba43 = p.betaa0Kv43 * gw::exp(-p.baKv43 * V);
+    const double bi14 = p.betai0Kv14 * gw::exp(p.biKv14 * V);
+    const double bi43 = p.betai0Kv43 * gw::exp(p.biKv43 * V);
+    struct Chain { double *q; double aa, ai, ba, bi, f1, f2, f3, f4, b1, b2, b3, b4; };
+    const Chain chains[2] = {
+        {Q.Kv14, aa14, ai14, ba14, bi14, p.f1Kv14, p.f2Kv14, p.f3Kv14, p.f4Kv14, p.b1Kv14, p.b2Kv14, p.b3Kv14, p.b4Kv14},
+        {Q.Kv43, aa43, ai43, ba43, bi43, p.f1Kv43, p.f2Kv43, p.f3Kv43, p.f4Kv43, p.b1Kv43, p.b2Kv43, p.b3Kv43, p.b4Kv43}};
+    for (int ci = 0; ci < 2; ++ci) {
+        const Chain &h = chains[ci];
+        double *q = h.q;
+#define A(i, j) q[(i) * 10 + (j)]
+        A(0, 1) = 4 * h.aa;            A(0, 5) = h.bi;                A(0, 0) = -(A(0, 1) + A(0, 5));
+        A(1, 0) = h.ba;                A(1, 2) = 3 * h.aa;            A(1, 6) = h.f1 * h.bi;
+        A(1, 1) = -(A(1, 0) + A(1, 2) + A(1, 6));
+        A(2, 1) = 2 * h.ba;            A(2, 3) = 2 * h.aa;            A(2, 7) = h.f2 * h.bi;
+        A(2, 2) = -(A(2, 1) + A(2, 3) + A(2, 7));
+        A(3, 2) = 3 * h.ba;            A(3, 4) = h.aa;                A(3, 8) = h.f3 * h.bi;
+        A(3, 3) = -(A(3, 2) + A(3, 4) + A(3, 8));
+        A(4, 3) = 4 * h.ba;            A(4, 9) = h.f4 * h.bi;         A(4, 4) = -(A(4, 3) + A(4, 9));
+        A(5, 6) = 4 * h.aa * h.b1;     A(5, 0) = h.ai;                A(5, 5) = -(A(5, 6) + A(5, 0));
+        A(6, 5) = h.ba / h.f1;         A(6, 7) = 3 * h.aa * h.b2 / h.b1;  A(6, 1) = h.ai / h.b1;
+        A(6, 6) = -(A(6, 5) + A(6, 7) + A(6, 1));
+        A(7, 6) = 2 * h.ba * h.f1 / h.f2;  A(7, 8) = 2 * h.aa * h.b3 / h.b2;  A(7, 2) = h.ai / h.b2;
+        A(7, 7) = -(A(7, 6) + A(7, 8) + A(7, 2));
+        A(8, 7) = 3 * h.ba * h.f2 / h.f3;  A(8, 9) = h.aa * h.b4 / h.b3;      A(8, 3) = h.ai / h.b3;
+        A(8, 8) = -(A(8, 7) + A(8, 9) + A(8, 3));
+        A(9, 8) = 4 * h.ba * h.f3 / h.f4;  A(9, 4) = h.ai / h.b4;             A(9, 9) = -(A(9, 8) + A(9, 4));
+#undef A
+    }
+}
+
+// Currents of one step, in the order of GW::Currents (GW.hpp:25-42).
+struct Currents {
+    double INa, INab, INaCa, INaK, IKr, IKs, Ito1, Ito2, IK1, IKp, ICaL, ICab, IpCa, Jup, Jtr_tot, Jxfer_tot;
+};
+
+GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT,
+                             double expmVF_RT, double dt, double Istim, const FluxSums &S, gw_globals_t &dg,
+                             Currents &cur, const QMatrices &Q)
+{
+    const double V = g.V;
+    bool ok = true;
+    const double ENa = nernst(g.Nai, p.Nao, c.RT_F, 1.0, ok);
+    const double EK = nernst(g.Ki, p.Ko, c.RT_F, 1.0, ok);
+    const double ECa = nernst(g.Ki, p.Ko, c.RT_F, 2.0, ok);  // K-based, as GW.tpp:176
+    if (!ok) return GW_E_NONPOSITIVE_CONC;
+
+    cur.INa = p.GNa * cube(g.m) * g.h * g.j * (V - ENa);                       // common.hpp:89-92
+    cur.INab = p.GNab * (V - ENa);                                             // common.hpp:100-101
+    {   // common.hpp:163-168
+        const double e1 = gw::exp(p.eta * VF_RT);
+        const double e2 = e1 * expmVF_RT;
+        cur.INaCa = c.INaCa_const * (e1 * cube(g.Nai) * p.Cao - e2 * c.Nao3 * g.Cai) / (1.0 + p.ksat * e2);
+    }
+    {   // common.hpp:121-124, 140-143
+        const double fNaK = 1.0 / (1.0 + 0.1245 * gw::exp(-0.1 * VF_RT) + 0.0365 * c.sigma * expmVF_RT);
+        cur.INaK = c.INaK_const * fNaK / (1.0 + gw::pow(p.KmNai / g.Nai, 1.5));
+    }
+    {   // GW_utils.hpp:273-279
+        const double RKr = 1.0 / (1.0 + 1.4945 * gw::exp(0.0446 * V));
+        cur.IKr = p.GKr * c.sqrtKo * RKr * g.Kr[3] * (V - EK) * 0.5;
+    }
+    {   // GW_utils.hpp:282-291
+        const double EKs = gw::log((p.Ko + 0.01833 * p.Nao) / (g.Ki + 0.01833 * g.Nai)) * c.RT_F;
+        cur.IKs = p.GKs * sq(g.xKs) * (V - EKs);
+    }
+    double IKv14;
+    {   // GW_utils.hpp:299-303 with exp_term = expmVF_RT (GW.tpp:185); limit at V == 0
+        const double e = expmVF_RT;
+        double mm;
+        if (VF_RT == 0.0) mm = (c.PKv14_Csc) * kFaraday * g.Kv14[4];
+        else mm = (c.PKv14_Csc) * (kFaraday * VF_RT) * g.Kv14[4] / (1 - e);
+        IKv14 = mm * ((g.Ki - p.Ko * e) + 0.02 * (g.Nai - p.Nao * e)) * 1.0e9;
+    }
+    const double IKv43 = p.GKv43 * g.Kv43[4] * (V - EK);                       // GW_utils.hpp:294-297
+    cur.Ito1 = IKv14 + IKv43;
+    if (VF_RT == 0.0) cur.Ito2 = S.ClCh * c.Ito2_const * (p.Clo - p.Clcyto);    // limit of GW_utils.hpp:330
+    else cur.Ito2 = S.ClCh * c.Ito2_const * VF_RT * (p.Clcyto * expmVF_RT - p.Clo) / (expmVF_RT - 1.0);
+    {   // GW_utils.hpp:306-312
+        const double K1inf = 1.0 / (2.0 + gw::exp(1.5 * (V - EK) * c.F_RT));
+        cur.IK1 = p.GK1 * K1inf * c.IK1_const * (V - EK);
+    }
+    {   // GW_utils.hpp:315-319
+        const double Kp = 1. / (1. + gw::exp((7.488 - V) / 5.98));
+        cur.IKp = p.GKp * Kp * (V - EK);
+    }
+    cur.ICaL = S.JLCC * c.ICaL_const;                                           // GW_utils.hpp:322-325
+    cur.ICab = p.GCab * (V - ECa);
+    cur.IpCa = p.IpCamax * (g.Cai / (p.KmpCa + g.Cai));                         // common.hpp:109-110
+    {   // GW_utils.hpp:339-344
+        const double f = gw::pow(g.Cai / p.Kmf, p.Hf);
+        const double r = gw::pow(g.CaNSR / p.Kmr, p.Hr);
+        cur.Jup = (p.Vmaxf * f - p.Vmaxr * r) / (1.0 + f + r);
+    }
+    cur.Jtr_tot = S.Jtr * c.CRU_factor;                                         // GW_utils.hpp:354-355
+    cur.Jxfer_tot = S.Jxfer * c.CRU_factor;
+    double beta_cyto;
+    {   // GW_utils.hpp:347-351
+        const double x = p.KCMDN + g.Cai;
+        beta_cyto = 1.0 / (1.0 + c.CMDN_const / sq(x));
+    }
+    // GW.tpp:201-211
+    dg.CaLTRPN = p.kLTRPNp * g.Cai * (p.LTRPNtot - g.CaLTRPN) - p.kLTRPNm * g.CaLTRPN;
+    dg.CaHTRPN = p.kHTRPNp * g.Cai * (p.HTRPNtot - g.CaHTRPN) - p.kHTRPNm * g.CaHTRPN;
+    dg.Nai = -dt * c.CSA_FVcyto * (cur.INa + cur.INab + 3 * cur.INaCa + 3 * cur.INaK);
+    dg.Ki = -dt * c.CSA_FVcyto * (cur.IKr + cur.IKs + cur.Ito1 + cur.IK1 + cur.IKp - 2 * cur.INaK);
+    dg.Cai = dt * beta_cyto *
+             (-0.5 * c.CSA_FVcyto * (cur.ICab + cur.IpCa - 2 * cur.INaCa) + c.VSS_Vcyto * cur.Jxfer_tot - cur.Jup -
+              (dg.CaLTRPN + dg.CaHTRPN));
+    dg.CaNSR = dt * (c.Vcyto_VNSR * cur.Jup - c.VJSR_VNSR * cur.Jtr_tot);
+    dg.CaLTRPN *= dt;
+    dg.CaHTRPN *= dt;
+    dg.V = dt * (Istim - (cur.INa + cur.ICaL + cur.IKr + cur.IKs + cur.Ito1 + cur.IK1 + cur.IKp + cur.Ito2 +
+                          cur.INaK + cur.INaCa + cur.IpCa + cur.ICab + cur.INab));
+    // GW.tpp:162-168
+    dg.m = dt * (alpha_m(V) * (1.0 - g.m) - beta_m(V) * g.m);
+    dg.h = dt * (alpha_h(V) * (1.0 - g.h) - beta_h(V) * g.h);
+    dg.j = dt * (alpha_j(V) * (1.0 - g.j) - beta_j(V) * g.j);
+    dg.xKs = dt * (XKsinf(V) - g.xKs) * tauXKs_inv(V);
+    // GW.tpp:126-133
+    {
+        const double *q = Q.Kr;
+        const double *x = g.Kr;
+#define KR(i, j) q[(i) * 5 + (j)]
+        dg.Kr[0] = dt * (KR(1, 0) * x[1] + KR(0, 0) * x[0]);
+        dg.Kr[1] = dt * (KR(0, 1) * x[0] + KR(2, 1) * x[2] + KR(1, 1) * x[1]);
+        dg.Kr[2] = dt * (KR(1, 2) * x[1] + KR(3, 2) * x[3] + KR(4, 2) * x[4] + KR(2, 2) * x[2]);
+        dg.Kr[3] = dt * (KR(2, 3) * x[2] + KR(4, 3) * x[4] + KR(3, 3) * x[3]);
+        dg.Kr[4] = dt * (KR(2, 4) * x[2] + KR(3, 4) * x[3] + KR(4, 4) * x[4]);
+#undef KR
+    }
+    // GW.tpp:135-160
+    const double *qs[2] = {Q.Kv14, Q.Kv43};
+    const double *ys[2] = {g.Kv14, g.Kv43};
+    double *ds[2] = {dg.Kv14, dg.Kv43};
+    for (int ci = 0; ci < 2; ++ci) {
+        const double *q = qs[ci];
+        const double *y = ys[ci];
+        double *d = ds[ci];
+#define A(i, j) q[(i) * 10 + (j)]
+        d[0] = dt * (A(1, 0) * y[1] + A(5, 0) * y[5] + A(0, 0) * y[0]);
+        d[1] = dt * (A(0, 1) * y[0] + A(2, 1) * y[2] + A(6, 1) * y[6] + A(1, 1) * y[1]);
+        d[2] = dt * (A(1, 2) * y[1] + A(3, 2) * y[3] + A(7, 2) * y[7] + A(2, 2) * y[2]);
+        d[3] = dt * (A(2, 3) * y[2] + A(4, 3) * y[4] + A(8, 3) * y[8] + A(3, 3) * y[3]);
+        d[4] = dt * (A(3, 4) * y[3] + A(9, 4) * y[9] + A(4, 4) * y[4]);
+        d[5] = dt * (A(6, 5) * y[6] + A(0, 5) * y[0] + A(5, 5) * y[5]);
+        d[6] = dt * (A(5, 6) * y[5] + A(7, 6) * y[7] + A(1, 6) * y[1] + A(6, 6) * y[6]);
+        d[7] = dt * (A(6, 7) * y[6] + A(8, 7) * y[8] + A(2, 7) * y[2] + A(7, 7) * y[7]);
+        d[8] = dt * (A(7, 8) * y[7] + A(9, 8) * y[9] + A(3, 8) * y[3] + A(8, 8) * y[8]);
+        d[9] = dt * (A(8, 9) * y[8] + A(4, 9) * y[4] + A(9, 9) * y[9]);
+#undef A
+    }
+    return GW_OK;
+}
+
+// globals += dGlobals (GW.tpp:252-271); V is held when clamped.
+GW_HD void globals_apply(gw_globals_t &g, const gw_globals_t &dg, bool clamp)
+{
+    if (!clamp) g.V += dg.V;
+    g.Nai += dg.Nai;
+    g.Ki += dg.Ki;
+    g.Cai += dg.Cai;
+    g.CaNSR += dg.CaNSR;
+    g.CaLTRPN += dg.CaLTRPN;
+    g.CaHTRPN += dg.CaHTRPN;
+    g.m += dg.m;
+    g.h += dg.h;
+    g.j += dg.j;
+    g.xKs += dg.xKs;
+    for (int i = 0; i < 5; ++i) g.Kr[i] += dg.Kr[i];
+    for (int i = 0; i < 10; ++i) {
+        g.Kv14[i] += dg.Kv14[i];
+        g.Kv43[i] += dg.Kv43[i];
+    }
+}
+
+}  // namespace gw

@@ -173,17 +173,23 @@ typedef struct gw_cell_desc_t {
     gw_protocol_t protocol;
 } gw_cell_desc_t;
 
-/* Per-cell observables tracked on the device every step (all FP64). */
+/*
+ * Per-cell observables tracked on the device every step (all FP64).
+ * APD90 of a beat: from the upward crossing of -40 mV to the downward crossing
+ * of V_rest + 0.1 (V_peak - V_rest), both interpolated linearly within a step.
+ */
 typedef struct gw_cell_obs_t {
-    double peak_V, min_V, peak_Cai, peak_ICaL_neg;   /* running extrema since last reset */
-    double sum_open_RyR_frac;                        /* sum over steps of mean RyR open fraction */
-    double int_Jrel;                                 /* integral of the release flux over time (mM) */
-    double n_steps;                                  /* steps accumulated since reset */
-    double apd_v_rest, apd_t_up, apd_t_peak, apd90;  /* APD90 tracker of the current beat */
-    double apd90_last, apd90_prev;                   /* APD90 of the last two completed beats */
+    double peak_V, min_V, peak_Cai, peak_ICaL_neg;   /* running extrema since the last reset */
+    double sum_open_RyR_frac;                        /* sum over steps of the RyR open fraction */
+    double int_Jrel;                                 /* integral over time of CRU_factor * sum Jrel (mM) */
+    double n_steps;                                  /* steps accumulated since the last reset */
+    double v_prev;                                   /* V of the previous step */
+    double apd_v_rest;                               /* minimum V since the last repolarisation */
+    double apd_v_peak;                               /* peak V of the beat in progress */
+    double apd_t_up;                                 /* upstroke time of the beat in progress (-1: none) */
+    double apd90_last, apd90_prev;                   /* APD90 of the last two completed beats (-1: none) */
     double alt_sum, alt_count;                       /* sum and count of |APD90_k - APD90_{k-1}| */
     double beats_done;
-    double _reserved[1];
 } gw_cell_obs_t;
 
 typedef struct gw_ctx gw_ctx;
@@ -197,6 +203,13 @@ void gw_globals_default(gw_globals_t *g);
 int gw_prng_from_name(const char *name);
 /* Library version (GW_CAPI_VERSION). */
 int gw_version(void);
+/*
+ * Seeded rest state of n CRUs on the host, identical to the state a batch cell
+ * with the same seed starts from. Replaces the random CRUState constructor
+ * (src/includes/GW_utils.tpp:210-218, initialise_* at :5-64), whose
+ * std::random_device seed is not reproducible. No GPU is used.
+ */
+int gw_rest_state(int32_t n_cru, uint64_t seed, int32_t prng_id, gw_cru_state_t *out);
 
 /* Create / destroy a device context (one HIP stream + scratch) on `device`. */
 int gw_ctx_create(gw_ctx **out, int device, char *err, size_t errlen);
@@ -257,6 +270,12 @@ int gw_batch_get_trace(gw_batch *b, double *out, int64_t max_rows, int64_t *rows
 /* Per-CRU replay digest: number of SSA events and a 64-bit hash of the event sequence. */
 int gw_batch_get_event_digest(gw_batch *b, int32_t cell, uint64_t *count, uint64_t *hash,
                               char *err, size_t errlen);
+/*
+ * Replay controls (no reference counterpart; used for bit-exact replay against
+ * a CPU restatement): frozen != 0 keeps every cell's globals fixed so only the
+ * SSA advances; digest != 0 keeps the per-CRU event digests.
+ */
+int gw_batch_set_mode(gw_batch *b, int32_t frozen, int32_t digest);
 /* Total SSA loop iterations executed (diagnostic for the flop count per CRU-step). */
 int gw_batch_get_iterations(gw_batch *b, uint64_t *iters);
 /*

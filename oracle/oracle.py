@@ -100,7 +100,7 @@ _LIBS: dict[str, C.CDLL] = {}
 
 
 def lib(kind: str = "strict") -> C.CDLL:
-    name = {"strict": "liboracle.so", "omp": "liboracle_omp.so"}[kind]
+    name = {"strict": "liboracle.so", "par": "liboracle_par.so", "omp": "liboracle_omp.so"}[kind]
     if kind in _LIBS:
         return _LIBS[kind]
     path = os.path.join(BUILD, name)

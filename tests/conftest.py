@@ -28,3 +28,24 @@ def golden():
 
     with open(os.path.join(ROOT, "tests", "golden", "ref_vectors.json"), encoding="utf-8") as f:
         return json.load(f)
+
+
+def _load_capi():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("gw_capi_binding", os.path.join(ROOT, "gw-model_amd", "capi.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def gwcapi():
+    return _load_capi()
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx(gwcapi):
+    ctx = gwcapi.Context(0)
+    yield ctx
+    ctx.close()

@@ -1,0 +1,214 @@
+"""The drop-in surface on the CPU: C-ABI exports and host entry points, the
+GreensteinWinslow module's classes and defaults (src/pyGW.cpp:160-344), and
+the Python wrappers (src/parameters.py, src/model.py, src/utils.py). No GPU
+compute is called here."""
+import ctypes as C
+import importlib
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import GwParams, OracleModel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    return importlib.import_module("gw-model_amd")
+
+
+@pytest.fixture(scope="module")
+def gw(pkg):
+    import build.GreensteinWinslow as m  # resolved through the package's sys.path entry
+
+    return m
+
+
+def test_library_exports_every_declared_symbol(gwcapi):
+    names = gwcapi.exported_symbols()
+    assert len(names) >= 25
+    L = C.CDLL(gwcapi.LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", gwcapi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (gw_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_param_defaults_agree_everywhere(gwcapi, gw):
+    p = gwcapi.Params()
+    gwcapi.lib().gw_params_default(C.byref(p))
+    ref = GwParams.default()
+    py = gw.GWParameters()
+    names = [n for n, _ in p._fields_ if not n.startswith("_")]
+    assert len(names) == 115
+    for n in names:
+        assert getattr(p, n) == getattr(ref, n) == getattr(py, n), n
+    assert isinstance(py.NCaRU, int) and py.NCaRU == 12500
+
+
+def test_globals_defaults(gwcapi, gw):
+    g = gwcapi.default_globals()
+    assert g.V == -91.382 and g.CaLTRPN == 8.98282e-3  # GW_utils.hpp:237
+    s = gw.GWGlobalState()
+    assert s.CaLTRPN == 8.9282e-3  # the binding's own default (pyGW.cpp:37)
+    assert s.XKr.shape == (5,) and s.XKv14.shape == (10,) and s.XKv43.shape == (10,)
+    assert s.XKv14[0] == 0.722328 and s.XKv43[0] == 0.953060
+    s.XKr[1] = 0.25
+    assert s.XKr[1] == 0.25  # element writes persist, as with the reference's Eigen members
+    with pytest.raises(ValueError):
+        s.XKv14 = np.zeros(9)
+
+
+@pytest.mark.parametrize("prng", range(6))
+def test_host_rest_state_matches_oracle(gwcapi, prng):
+    n, seed = 257, 99 + prng
+    out = {"CaSS": np.zeros((n, 4)), "CaJSR": np.zeros(n), "LCC": np.zeros((n, 4), np.int32),
+           "LCC_inactivation": np.zeros((n, 4), np.int32), "RyR": np.zeros((n, 4, 6), np.int32),
+           "ClCh": np.zeros((n, 4), np.int32), "JLCC": np.zeros((n, 4))}
+    cs = gwcapi.CruState(*[out[k].ctypes.data for k in ("CaSS", "CaJSR", "LCC", "LCC_inactivation", "RyR", "ClCh", "JLCC")])
+    L = gwcapi.lib()
+    L.gw_rest_state.argtypes = [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(gwcapi.CruState)]
+    assert L.gw_rest_state(n, seed, prng, C.byref(cs)) == 0
+    m = OracleModel(n, seed=seed, prng=prng)
+    m.sample_rest_state()
+    ref = m.state()
+    for k in out:
+        assert np.array_equal(out[k], ref[k]), k
+    # the rest-state occupancies are those of the reference sampler's weights
+    assert set(np.unique(out["LCC"])) <= {1, 2, 7}
+    assert np.all(out["RyR"].sum(axis=2) == 5)
+
+
+def test_rest_state_statistics(gwcapi):
+    n = 20000
+    out = {"CaSS": np.zeros((n, 4)), "CaJSR": np.zeros(n), "LCC": np.zeros((n, 4), np.int32),
+           "LCC_inactivation": np.zeros((n, 4), np.int32), "RyR": np.zeros((n, 4, 6), np.int32),
+           "ClCh": np.zeros((n, 4), np.int32)}
+    cs = gwcapi.CruState(*[out[k].ctypes.data for k in ("CaSS", "CaJSR", "LCC", "LCC_inactivation", "RyR", "ClCh")], None)
+    L = gwcapi.lib()
+    L.gw_rest_state.argtypes = [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(gwcapi.CruState)]
+    L.gw_rest_state(n, 1, 0, C.byref(cs))
+    N = 4 * n
+    for val, p in ((1, 0.958), (2, 0.038), (7, 0.004)):
+        assert abs((out["LCC"] == val).sum() / N - p) < 5 * np.sqrt(p * (1 - p) / N)
+    assert abs(out["LCC_inactivation"].mean() - 0.9425) < 5 * np.sqrt(0.9425 * 0.0575 / N)
+    assert abs(out["ClCh"].mean() - 0.002) < 5 * np.sqrt(0.002 * 0.998 / N)
+    M = 5 * N
+    for k, p in ((0, 0.609), (4, 0.1955), (5, 0.1955)):
+        assert abs(out["RyR"][..., k].sum() / M - p) < 5 * np.sqrt(p * (1 - p) / M)
+
+
+def test_prng_names(gwcapi):
+    L = gwcapi.lib()
+    for i, name in enumerate(gwcapi.PRNG_NAMES):
+        assert L.gw_prng_from_name(name.encode()) == i
+    assert L.gw_prng_from_name(b"mt19937") == -1
+
+
+def test_binding_classes(gw):
+    assert repr(gw.GWParameters()) == "Greenstein and Winslow Model Parameters"
+    v = gw.GWVariables(7, 3, 2.5)
+    assert repr(v) == "Greenstein and Winslow model solution over the interval [0, 2.500000] with 7 CRUs"
+    assert v.V.shape == (3,) and v.XKr.shape == (3, 5) and v.XKv14.shape == (3, 10)
+    assert v.CaJSR.shape == (3, 7) and v.CaSS.shape == (3, 7, 4) and v.RyR.shape == (3, 7, 4, 6)
+    assert v.LCC.dtype == np.int32 and v.CaSS.dtype == np.float64
+    c = gw.GWCRUState(11)
+    assert c.CaSS.shape == (11, 4) and c.CaJSR.shape == (11,) and c.RyR.shape == (11, 4, 6)
+    r0 = gw.rest_state(11, 0)
+    assert np.array_equal(c.LCC, r0.LCC) and np.array_equal(c.RyR, r0.RyR)  # seeded rest state, not garbage
+    c.LCC = np.full((11, 4), 2)
+    assert c.LCC.dtype == np.int32 and np.all(c.LCC == 2)
+
+
+def test_run_argument_errors(gw):
+    p = gw.GWParameters()
+    c = gw.GWCRUState(4)
+    with pytest.raises(ValueError):
+        gw.run(p, 4, 1e-3, 10, lambda t: 0.0, 1, c, PRNG="not-an-engine")
+    with pytest.raises(ValueError):
+        gw.run(p, 4, 1e-3, 10, lambda t: 0.0, 0, c)
+    with pytest.raises(ValueError):
+        gw.run(p, 5, 1e-3, 10, lambda t: 0.0, 1, c)  # init_crus has 4 CRUs
+
+
+def test_parameters_wrapper(pkg):
+    P = pkg.GWParameters
+    p = P(NCaRU_sim=500, g=1.8, omega=0.0)
+    assert p.NCaRU_sim == 500 and p.g == 1.8 and p.omega == 0.0
+    with pytest.raises(ValueError):
+        P(T=0.0)
+    with pytest.raises(ValueError):
+        P(k12=-1.0)
+    with pytest.raises(TypeError):
+        P(NCaRU_sim=10.0)
+    with pytest.raises(ValueError):
+        P(not_a_parameter=1)
+    d = p.to_dict()
+    assert len(d) == 116
+    q = P.from_dict(d)
+    assert q.to_dict() == d
+
+
+def test_model_wrapper_validation(pkg):
+    m = pkg.GWModel(parameters=pkg.GWParameters(NCaRU_sim=8))
+    assert "xoshiro256**" in m.GWModel.PRNG_options() if hasattr(m, "GWModel") else pkg.GWModel.PRNG_options()
+    for kw in ({"step_size": 0.0, "num_steps": 10}, {"step_size": 1e-3, "num_steps": 0},
+               {"step_size": 1e-3, "num_steps": 10, "record_every": 0}):
+        with pytest.raises(ValueError):
+            m.simulate(**kw)
+    with pytest.raises(ValueError, match="invalid argument for PRNG"):
+        m.simulate(1e-3, 10, PRNG="bogus")
+
+
+def test_ryr_int16_encoding(pkg):
+    R = np.array([[5, 0, 0, 0, 0, 0], [0, 1, 2, 1, 1, 0], [0, 1, 4, -1, 0, 0], [1, 0, 0, 0, 5, -1]])
+    code = pkg.RyR_state_to_int16(R)
+    assert code.dtype == np.int16
+    assert code.tolist() == [5, 6 + 72 + 216 + 1296, 6 + 36 * 3, 1 + 1296 * 4]
+
+
+def test_numpy_validators_keep_reference_semantics(pkg):
+    # raise only when every element violates the bound (src/utils.py:167,173)
+    pkg.assert_gt_numpy(np.array([-1.0, 1.0]), 0, "x", strict=True)
+    with pytest.raises(ValueError):
+        pkg.assert_gt_numpy(np.array([-1.0, 0.0]), 0, "x", strict=True)
+
+
+def test_posthoc_ical_ito2_closed_forms(pkg):
+    p = pkg.GWParameters(NCaRU_sim=2)
+    V = np.array([-80.0, 10.0])
+    LCC = np.ones((2, 2, 4), np.int32)
+    y = np.ones((2, 2, 4), np.int32)
+    CaSS = np.full((2, 2, 4), 1e-4)
+    assert np.all(pkg.ICaL(V, LCC, y, CaSS, p) == 0)
+    LCC[:, 0, 0] = 6
+    one = pkg.ICaL(V, LCC, y, CaSS, p)
+    LCC[:, 1, 2] = 12
+    two = pkg.ICaL(V, LCC, y, CaSS, p)
+    assert np.allclose(two, 2 * one) and np.all(one < 0)  # inward, linear in open channels
+    y[:, 1, 2] = 0
+    assert np.allclose(pkg.ICaL(V, LCC, y, CaSS, p), one)  # voltage-inactivated channels carry no current
+    cl = np.zeros((2, 2, 4), np.int32)
+    assert np.all(pkg.Ito2(V, cl, p) == 0)
+    cl[:, 0, 1] = 1
+    E_cl = 8.314 * p.T / 96.5 * np.log(p.Clcyto / p.Clo)
+    assert abs(pkg.Ito2(np.array([E_cl, E_cl]), cl, p)).max() < 1e-12
+
+
+def test_product_never_uses_the_oracle():
+    """The package neither imports, links nor loads anything under oracle/."""
+    pkg_dir = os.path.join(ROOT, "gw-model_amd")
+    pat = re.compile(r"(^\s*(from|import)\s+oracle)|liboracle|gw_oracle|OracleModel|oracle/", re.M)
+    for dirpath, _, files in os.walk(pkg_dir):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                text = open(os.path.join(dirpath, f), encoding="utf-8").read()
+                assert not pat.search(text), f
