@@ -92,17 +92,26 @@ __device__ void obs_update(gw_cell_obs_t &o, double V_old, double V, double t_ol
 
 // Derivatives and SSA constants of the step d.step at state d.g (the part of
 // euler_step before SSA(dt), GW.tpp:238-247 and 216-221).
+// Works on register copies of the cell's globals and constants; the results
+// are written back to the cell's dynamic state once.
 __device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, double dt, Currents &cur)
 {
-    d.Istim = protocol_istim(C, d);
-    if (d.clamp) d.g.V = protocol_vclamp(C, d.t);
-    const double VF_RT = d.g.V * C.c.F_RT;
+    const gw_params_t p = C.p;
+    const CellConsts c = C.c;
+    gw_globals_t g = d.g;
+    gw_globals_t dg;
+    const double Istim = protocol_istim(C, d);
+    if (d.clamp) g.V = protocol_vclamp(C, d.t);
+    const double VF_RT = g.V * c.F_RT;
     const double expm = gw::exp(-VF_RT);
-    QMatrices Q;
-    q_matrices(Q, C.p, d.g.V);
-    const int rc = global_derivatives(C.p, C.c, d.g, VF_RT, expm, dt, d.Istim, S, d.dg, cur, Q);
+    const int rc = global_derivatives(p, c, g, VF_RT, expm, dt, Istim, S, dg, cur);
     if (rc != GW_OK && d.status == 0) d.status = rc;
-    step_consts(d.k, C.p, C.c, d.g.V, expm, d.g.Cai, d.g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+    StepConsts k;
+    step_consts(k, p, c, g.V, expm, g.Cai, g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+    d.Istim = Istim;
+    d.g = g;
+    d.dg = dg;
+    d.k = k;
 }
 
 __device__ void sum_block_partials(const Partial *parts, int64_t first, int n, Partial &tot)
@@ -134,12 +143,13 @@ __device__ void partial_add(Partial &a, const Partial &b)
     a.err |= b.err;
 }
 
+template <int NT>
 __device__ void block_reduce(Partial *red, Partial mine)
 {
     const int tid = threadIdx.x;
     red[tid] = mine;
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (tid < s) partial_add(red[tid], red[tid + s]);
         __syncthreads();
     }
@@ -230,7 +240,7 @@ __global__ void k_init_cells(KArgs a)
             mine.open += ryr_get(w, 2) + ryr_get(w, 3);
         }
     }
-    block_reduce(red, mine);
+    block_reduce<kBlock>(red, mine);
     if (threadIdx.x == 0) {
         const Partial &t = red[0];
         FluxSums S{fx_to_double(t.jlcc), fx_to_double(t.jxfer), fx_to_double(t.jtr), static_cast<double>(t.clch)};
@@ -242,96 +252,126 @@ __global__ void k_init_cells(KArgs a)
         Currents cur;
         prepare_step(C, D, S, a.dt, cur);
         D.obs.v_prev = D.g.V;
-        a.counters[cell] = 0;
     }
 }
 
-// One Euler step of every cell.
-template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock) void k_ssa_step(KArgs a)
+__device__ __forceinline__ void load_cru(const KArgs &a, int64_t slot, Cru64 &c)
 {
-    __shared__ SsaParams sp;
-    __shared__ StepConsts kk;
+    const int64_t N = a.ntot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        c.ca[j] = a.CaSS[j * N + slot];
+        c.jl[j] = a.JLCC[j * N + slot];
+        c.ryr[j] = a.ryr[j * N + slot];
+    }
+    c.jsr = a.CaJSR[slot];
+    c.chan = a.chan[slot];
+    c.r0 = a.rng[slot];
+    c.r1 = a.rng[N + slot];
+    c.r2 = a.rng[2 * N + slot];
+    c.r3 = a.rng[3 * N + slot];
+}
+
+__device__ __forceinline__ void store_cru(const KArgs &a, int64_t slot, const Cru64 &c)
+{
+    const int64_t N = a.ntot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        a.CaSS[j * N + slot] = c.ca[j];
+        a.JLCC[j * N + slot] = c.jl[j];
+        a.ryr[j * N + slot] = c.ryr[j];
+    }
+    a.CaJSR[slot] = c.jsr;
+    a.chan[slot] = c.chan;
+    a.rng[slot] = c.r0;
+    a.rng[N + slot] = c.r1;
+    a.rng[2 * N + slot] = c.r2;
+    a.rng[3 * N + slot] = c.r3;
+}
+
+__device__ __forceinline__ void partial_zero(Partial &m)
+{
+    fx_zero(m.jlcc); fx_zero(m.jxfer); fx_zero(m.jtr); fx_zero(m.jrel);
+    m.clch = 0; m.open = 0; m.iters = 0; m.err = 0;
+}
+
+__device__ __forceinline__ void partial_from(Partial &m, const CruOut &o, uint64_t iters)
+{
+    m.err |= fx_add(m.jlcc, o.JLCC);
+    m.err |= fx_add(m.jxfer, o.Jxfer);
+    m.err |= fx_add(m.jtr, o.Jtr);
+    fx_add(m.jrel, o.Jrel);
+    m.clch = o.clch;
+    m.open = o.open;
+    m.iters = iters;
+    m.err |= o.err ? 4 : 0;
+}
+
+// Copy this step's LCC rate table into LDS (it is read with a per-lane index;
+// the cell's other constants are wave-uniform and are read as scalars).
+__device__ __forceinline__ void load_lcc_table(const CellDyn &D, LccTable &T)
+{
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&D.k.lcc);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(&T);
+    for (int w = threadIdx.x; w < static_cast<int>(sizeof(LccTable) / 8); w += blockDim.x) dst[w] = src[w];
+}
+
+// SSA(dt) of one step (GW_model::SSA, src/includes/GW.tpp:214-234) for the
+// CRUs of this block; the block's flux partial goes to partials[block].
+template <int PRNG, bool DIGEST>
+__global__ __launch_bounds__(kBlock, 4) void k_ssa_step(KArgs a)
+{
+    __shared__ LccTable T;
     __shared__ Partial red[kBlock];
-    __shared__ int is_last;
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
-    const int cell = a.block_cell[b];
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
     const DevCell &C = a.cells[cell];
-    CellDyn &D = a.dyn[cell];
-    {
-        const uint64_t *src = reinterpret_cast<const uint64_t *>(&C.sp);
-        uint64_t *dst = reinterpret_cast<uint64_t *>(&sp);
-        for (int w = tid; w < static_cast<int>(sizeof(SsaParams) / 8); w += kBlock) dst[w] = src[w];
-        const uint64_t *src2 = reinterpret_cast<const uint64_t *>(&D.k);
-        uint64_t *dst2 = reinterpret_cast<uint64_t *>(&kk);
-        for (int w = tid; w < static_cast<int>(sizeof(StepConsts) / 8); w += kBlock) dst2[w] = src2[w];
-    }
+    const CellDyn &D = a.dyn[cell];
+    const SsaParams &sp = C.sp;
+    const StepConsts &kk = D.k;
+    load_lcc_table(D, T);
     __syncthreads();
     const int64_t i = (b - C.blk0) * kBlock + tid;
     const bool active = (i < C.n_cru) && (D.status == 0);
     Partial mine;
-    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
-    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    partial_zero(mine);
     if (active) {
         const int64_t slot = C.slot0 + i;
-        const int64_t N = a.ntot;
         Cru64 c;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            c.ca[j] = a.CaSS[j * N + slot];
-            c.jl[j] = a.JLCC[j * N + slot];
-            c.ryr[j] = a.ryr[j * N + slot];
-        }
-        c.jsr = a.CaJSR[slot];
-        c.chan = a.chan[slot];
-        c.r0 = a.rng[slot];
-        c.r1 = a.rng[N + slot];
-        c.r2 = a.rng[2 * N + slot];
-        c.r3 = a.rng[3 * N + slot];
+        load_cru(a, slot, c);
         Digest dg{0, 0};
         if (DIGEST) {
             dg.count = a.dig[slot];
-            dg.hash = a.dig[N + slot];
+            dg.hash = a.dig[a.ntot + slot];
         }
-        const CruOut o = ssa_step<PRNG, DIGEST>(c, sp, kk, a.dt, dg);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            a.CaSS[j * N + slot] = c.ca[j];
-            a.JLCC[j * N + slot] = c.jl[j];
-            a.ryr[j * N + slot] = c.ryr[j];
-        }
-        a.CaJSR[slot] = c.jsr;
-        a.chan[slot] = c.chan;
-        a.rng[slot] = c.r0;
-        a.rng[N + slot] = c.r1;
-        a.rng[2 * N + slot] = c.r2;
-        a.rng[3 * N + slot] = c.r3;
+        const CruOut o = ssa_step<PRNG, DIGEST>(c, sp, kk, T, a.dt, dg, kk.step_lo);
+        store_cru(a, slot, c);
         if (DIGEST) {
             a.dig[slot] = dg.count;
-            a.dig[N + slot] = dg.hash;
+            a.dig[a.ntot + slot] = dg.hash;
         }
-        mine.err |= fx_add(mine.jlcc, o.JLCC);
-        mine.err |= fx_add(mine.jxfer, o.Jxfer);
-        mine.err |= fx_add(mine.jtr, o.Jtr);
-        fx_add(mine.jrel, o.Jrel);
-        mine.clch = o.clch;
-        mine.open = o.open;
-        mine.iters = o.iters;
-        mine.err |= o.err ? 4 : 0;
+        partial_from(mine, o, o.iters);
     }
-    block_reduce(red, mine);
-    if (tid == 0) {
-        a.partials[b] = red[0];
-        const uint32_t prev = __hip_atomic_fetch_add(&a.counters[cell], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == static_cast<uint32_t>(C.nblk - 1));
-    }
-    __syncthreads();
-    if (!is_last || tid != 0) return;
-    // ---- last block of the cell: close step n, prepare step n+1 ----
-    Partial tot;
-    sum_block_partials(a.partials, C.blk0, C.nblk, tot);
-    a.counters[cell] = 0;
+    block_reduce<kBlock>(red, mine);
+    if (tid == 0) a.partials[b] = red[0];
+}
+
+// Close the step of every cell (one 64-lane block per cell): the exact sum of
+// the cell's block partials, globals += dGlobals (GW.tpp:252-271), t += dt,
+// then the next step's derivatives and SSA constants, observables and trace.
+__global__ __launch_bounds__(kWave) void k_cell_close(KArgs a)
+{
+    __shared__ Partial red[kWave];
+    const int cell = blockIdx.x;
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    Partial mine;
+    partial_zero(mine);
+    for (int q = threadIdx.x; q < C.nblk; q += kWave) partial_add(mine, a.partials[C.blk0 + q]);
+    block_reduce<kWave>(red, mine);
+    if (threadIdx.x != 0) return;
+    const Partial &tot = red[0];
     if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
     if (D.status != 0) return;
     if (tot.err) {
@@ -368,84 +408,47 @@ __global__ __launch_bounds__(kBlock) void k_ssa_step(KArgs a)
 // with the cell's constant SSA constants (globals held fixed), so only the
 // SSA runs. The flux sums of the final step are reduced as in k_ssa_step.
 template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock) void k_ssa_frozen(KArgs a, int64_t step0, int64_t nsteps)
+__global__ __launch_bounds__(kBlock, 4) void k_ssa_frozen(KArgs a, int64_t step0, int64_t nsteps)
 {
-    __shared__ SsaParams sp;
-    __shared__ StepConsts kk;
+    __shared__ LccTable T;
     __shared__ Partial red[kBlock];
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
-    const int cell = a.block_cell[b];
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
     const DevCell &C = a.cells[cell];
-    CellDyn &D = a.dyn[cell];
-    {
-        const uint64_t *src = reinterpret_cast<const uint64_t *>(&C.sp);
-        uint64_t *dst = reinterpret_cast<uint64_t *>(&sp);
-        for (int w = tid; w < static_cast<int>(sizeof(SsaParams) / 8); w += kBlock) dst[w] = src[w];
-        const uint64_t *src2 = reinterpret_cast<const uint64_t *>(&D.k);
-        uint64_t *dst2 = reinterpret_cast<uint64_t *>(&kk);
-        for (int w = tid; w < static_cast<int>(sizeof(StepConsts) / 8); w += kBlock) dst2[w] = src2[w];
-    }
+    const CellDyn &D = a.dyn[cell];
+    const SsaParams &sp = C.sp;
+    const StepConsts &kk = D.k;
+    load_lcc_table(D, T);
     __syncthreads();
     const int64_t i = (b - C.blk0) * kBlock + tid;
     const bool active = (i < C.n_cru) && (D.status == 0);
     Partial mine;
-    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
-    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    partial_zero(mine);
     if (active) {
         const int64_t slot = C.slot0 + i;
-        const int64_t N = a.ntot;
         Cru64 c;
-        for (int j = 0; j < 4; ++j) {
-            c.ca[j] = a.CaSS[j * N + slot];
-            c.jl[j] = a.JLCC[j * N + slot];
-            c.ryr[j] = a.ryr[j * N + slot];
-        }
-        c.jsr = a.CaJSR[slot];
-        c.chan = a.chan[slot];
-        c.r0 = a.rng[slot];
-        c.r1 = a.rng[N + slot];
-        c.r2 = a.rng[2 * N + slot];
-        c.r3 = a.rng[3 * N + slot];
+        load_cru(a, slot, c);
         Digest dg{0, 0};
         if (DIGEST) {
             dg.count = a.dig[slot];
-            dg.hash = a.dig[N + slot];
+            dg.hash = a.dig[a.ntot + slot];
         }
-        StepConsts k = kk;
         CruOut o{};
         uint64_t iters = 0;
         for (int64_t s = 0; s < nsteps; ++s) {
-            k.step_lo = static_cast<uint32_t>((step0 + s) & 0xffff);
-            o = ssa_step<PRNG, DIGEST>(c, sp, k, a.dt, dg);
+            o = ssa_step<PRNG, DIGEST>(c, sp, kk, T, a.dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff));
             iters += o.iters;
             if (o.err) break;
         }
-        for (int j = 0; j < 4; ++j) {
-            a.CaSS[j * N + slot] = c.ca[j];
-            a.JLCC[j * N + slot] = c.jl[j];
-            a.ryr[j * N + slot] = c.ryr[j];
-        }
-        a.CaJSR[slot] = c.jsr;
-        a.chan[slot] = c.chan;
-        a.rng[slot] = c.r0;
-        a.rng[N + slot] = c.r1;
-        a.rng[2 * N + slot] = c.r2;
-        a.rng[3 * N + slot] = c.r3;
+        store_cru(a, slot, c);
         if (DIGEST) {
             a.dig[slot] = dg.count;
-            a.dig[N + slot] = dg.hash;
+            a.dig[a.ntot + slot] = dg.hash;
         }
-        mine.err |= fx_add(mine.jlcc, o.JLCC);
-        mine.err |= fx_add(mine.jxfer, o.Jxfer);
-        mine.err |= fx_add(mine.jtr, o.Jtr);
-        fx_add(mine.jrel, o.Jrel);
-        mine.clch = o.clch;
-        mine.open = o.open;
-        mine.iters = iters;
-        mine.err |= o.err ? 4 : 0;
+        partial_from(mine, o, iters);
     }
-    block_reduce(red, mine);
+    block_reduce<kBlock>(red, mine);
     if (tid == 0) a.partials[b] = red[0];
 }
 
@@ -675,6 +678,7 @@ void launch_step(gw_batch *B)
         hipLaunchKernelGGL((k_ssa_step<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
     else
         hipLaunchKernelGGL((k_ssa_step<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
+    hipLaunchKernelGGL(k_cell_close, dim3(B->ncells), dim3(kWave), 0, B->ctx->stream, B->a);
 }
 
 void launch_steps(gw_batch *B, int64_t n)
@@ -855,7 +859,7 @@ int gw_batch_destroy(gw_batch *B)
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
     void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.chan, B->a.ryr, B->a.rng, B->a.dig, B->a.partials,
-                    B->a.counters, B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
+                    B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : B->istim_dev)
@@ -935,7 +939,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.chan, N, err, errlen)) ||
         (rc = dmalloc(&a.ryr, 4 * N, err, errlen)) || (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, B->nblocks, err, errlen)) ||
-        (rc = dmalloc(&a.counters, ncells, err, errlen)) || (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
+        (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
         (rc = dmalloc(&B->d_block_cell, B->nblocks, err, errlen))) {
         gw_batch_destroy(B);
@@ -970,7 +974,6 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     if ((e = hipMemcpyAsync(B->d_cells, B->cells_h.data(), ncells * sizeof(DevCell), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(B->d_dyn, dyn.data(), ncells * sizeof(CellDyn), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(B->d_block_cell, B->block_cell_h.data(), B->nblocks * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
-    if ((e = hipMemsetAsync(a.counters, 0, ncells * sizeof(uint32_t), ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemsetAsync(a.iters_total, 0, sizeof(unsigned long long), ctx->stream)) != hipSuccess) return fail(e);
     hipLaunchKernelGGL(k_seed, dim3(B->nblocks), dim3(kBlock), 0, ctx->stream, a);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);

@@ -59,7 +59,6 @@ struct KArgs {
     uint64_t *rng;     // [4][ntot]
     uint64_t *dig;     // [2][ntot] (count, hash) when digests are kept
     Partial *partials; // [nblocks]
-    uint32_t *counters;// [ncells]
     double *trace;     // [cap][ncells][4]
     unsigned long long *iters_total;
     int64_t ntot;

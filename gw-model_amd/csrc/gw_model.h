@@ -62,15 +62,19 @@ GW_HD void cell_consts_init(CellConsts &c, const gw_params_t &p, int nCRU)
 // Constants the SSA reads during one step (frozen inside GW_model::SSA,
 // src/includes/GW.tpp:214-233). The LCC propensities of a subunit in state s
 // are tabulated so that (r0 + r1) + r2 = lccP[s] + lccQ[s] * CaSS bit for bit;
-// lccR holds r0, r1, r2 with the CaSS-proportional slot set to zero.
+// R0, R1 hold r0, r1 with the CaSS-proportional slot set to zero (r2 is
+// never needed on its own: the transition scan stops before comparing it).
+struct LccTable {
+    double P[13], Q[13], R0[13], R1[13];
+};
+
 struct StepConsts {
     double Cai, CaNSR;
     double inact0, inact1;   // (y == 0) ? yinf/tau : (1 - yinf)/tau  (GW_utils.hpp:381)
     double JLCC_mult, JLCC_exp;
-    double lccP[13], lccQ[13];
-    double lccR0[13], lccR1[13], lccR2[13];
     uint32_t step_lo;
     uint32_t _pad;
+    LccTable lcc;            // read per lane (indexed by the LCC state)
 };
 
 // GW::alphaLCC .. tauLCC (GW_utils.hpp:365-375)
@@ -113,15 +117,14 @@ GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c,
     r0[11] = c.omega_b4;      r1[11] = 4 * be * c.binv; r2[11] = p.f1;
     r0[12] = p.g1;
     for (int s = 0; s < 13; ++s) {
-        k.lccR0[s] = r0[s];
-        k.lccR1[s] = r1[s];
-        k.lccR2[s] = r2[s];
-        k.lccQ[s] = q[s];
+        k.lcc.R0[s] = r0[s];
+        k.lcc.R1[s] = r1[s];
+        k.lcc.Q[s] = q[s];
         // state 1 carries its CaSS term in r1: (r0 + g0 CaSS) + 0 == P + Q CaSS with P = r0.
         // states 2..5 carry it in r2: (r0 + r1) + Q CaSS. Others: (r0 + r1) + r2 + 0.
-        if (s == 1) k.lccP[s] = r0[s];
-        else if (s >= 2 && s <= 5) k.lccP[s] = r0[s] + r1[s];
-        else k.lccP[s] = r0[s] + r1[s] + r2[s];
+        if (s == 1) k.lcc.P[s] = r0[s];
+        else if (s >= 2 && s <= 5) k.lcc.P[s] = r0[s] + r1[s];
+        else k.lcc.P[s] = r0[s] + r1[s] + r2[s];
     }
 }
 
@@ -171,64 +174,56 @@ GW_HD double tauXKs_inv(double V)
            0.000131 * (V - 10.0) / (gw::exp(0.0687 * (V - 10.0)) - 1.0);
 }
 
-struct QMatrices {
-    double Kr[25], Kv14[100], Kv43[100];
-};
-
-GW_HD void q_matrices(QMatrices &Q, const gw_params_t &p, double V)
+// IKr chain: generator entries of update_QKr (GW.tpp:25-44) and the
+// derivatives of update_Kr_derivatives (GW.tpp:126-133), in registers.
+GW_HD void kr_chain(const gw_params_t &p, double V, const double x[5], double dt, double d[5])
 {
-    double *Kr = Q.Kr;
-#define KR(i, j) Kr[(i) * 5 + (j)]
-    KR(1, 2) = p.Kf;
-    KR(2, 1) = p.Kb;
-    KR(0, 1) = 0.0069 * gw::exp(0.0272 * V);
-    KR(0, 0) = -KR(0, 1);
-    KR(1, 0) = 0.0227 * gw::exp(-0.0431 * V);
-    KR(1, 1) = -(KR(1, 0) + KR(1, 2));
-    KR(2, 3) = 0.0218 * gw::exp(0.0262 * V);
-    KR(2, 4) = 1.29e-5 * gw::exp(2.71e-6 * V);
-    KR(2, 2) = -(KR(2, 1) + KR(2, 3) + KR(2, 4));
-    KR(3, 2) = 0.0009 * gw::exp(-0.0269 * V);
-    KR(3, 4) = 0.0622 * gw::exp(0.0120 * V);
-    KR(3, 3) = -(KR(3, 2) + KR(3, 4));
-    KR(4, 3) = 0.0059 * gw::exp(-0.0443 * V);
-    KR(4, 2) = KR(3, 2) * KR(4, 3) * KR(2, 4) / (KR(2, 3) * KR(3, 4));
-    KR(4, 4) = -(KR(4, 2) + KR(4, 3));
-#undef KR
-    const double aa14 = p.alphaa0Kv14 * gw::exp(p.aaKv14 * V);
-    const double aa43 = p.alphaa0Kv43 * gw::exp(p.aaKv43 * V);
-    const double ai14 = p.alphai0Kv14 * gw::exp(-p.aiKv14 * V);
-    const double ai43 = p.alphai0Kv43 * gw::exp(-p.aiKv43 * V);
-    const double ba14 = p.betaa0Kv14 * gw::exp(-p.baKv14 * V);
-    const double ba43 = p.betaa0Kv43 * gw::exp(-p.baKv43 * V);
-    const double bi14 = p.betai0Kv14 * gw::exp(p.biKv14 * V);
-    const double bi43 = p.betai0Kv43 * gw::exp(p.biKv43 * V);
-    struct Chain { double *q; double aa, ai, ba, bi, f1, f2, f3, f4, b1, b2, b3, b4; };
-    const Chain chains[2] = {
-        {Q.Kv14, aa14, ai14, ba14, bi14, p.f1Kv14, p.f2Kv14, p.f3Kv14, p.f4Kv14, p.b1Kv14, p.b2Kv14, p.b3Kv14, p.b4Kv14},
-        {Q.Kv43, aa43, ai43, ba43, bi43, p.f1Kv43, p.f2Kv43, p.f3Kv43, p.f4Kv43, p.b1Kv43, p.b2Kv43, p.b3Kv43, p.b4Kv43}};
-    for (int ci = 0; ci < 2; ++ci) {
-        const Chain &h = chains[ci];
-        double *q = h.q;
-#define A(i, j) q[(i) * 10 + (j)]
-        A(0, 1) = 4 * h.aa;            A(0, 5) = h.bi;                A(0, 0) = -(A(0, 1) + A(0, 5));
-        A(1, 0) = h.ba;                A(1, 2) = 3 * h.aa;            A(1, 6) = h.f1 * h.bi;
-        A(1, 1) = -(A(1, 0) + A(1, 2) + A(1, 6));
-        A(2, 1) = 2 * h.ba;            A(2, 3) = 2 * h.aa;            A(2, 7) = h.f2 * h.bi;
-        A(2, 2) = -(A(2, 1) + A(2, 3) + A(2, 7));
-        A(3, 2) = 3 * h.ba;            A(3, 4) = h.aa;                A(3, 8) = h.f3 * h.bi;
-        A(3, 3) = -(A(3, 2) + A(3, 4) + A(3, 8));
-        A(4, 3) = 4 * h.ba;            A(4, 9) = h.f4 * h.bi;         A(4, 4) = -(A(4, 3) + A(4, 9));
-        A(5, 6) = 4 * h.aa * h.b1;     A(5, 0) = h.ai;                A(5, 5) = -(A(5, 6) + A(5, 0));
-        A(6, 5) = h.ba / h.f1;         A(6, 7) = 3 * h.aa * h.b2 / h.b1;  A(6, 1) = h.ai / h.b1;
-        A(6, 6) = -(A(6, 5) + A(6, 7) + A(6, 1));
-        A(7, 6) = 2 * h.ba * h.f1 / h.f2;  A(7, 8) = 2 * h.aa * h.b3 / h.b2;  A(7, 2) = h.ai / h.b2;
-        A(7, 7) = -(A(7, 6) + A(7, 8) + A(7, 2));
-        A(8, 7) = 3 * h.ba * h.f2 / h.f3;  A(8, 9) = h.aa * h.b4 / h.b3;      A(8, 3) = h.ai / h.b3;
-        A(8, 8) = -(A(8, 7) + A(8, 9) + A(8, 3));
-        A(9, 8) = 4 * h.ba * h.f3 / h.f4;  A(9, 4) = h.ai / h.b4;             A(9, 9) = -(A(9, 8) + A(9, 4));
-#undef A
-    }
+    const double q12 = p.Kf, q21 = p.Kb;   // initialise_QKr (GW.hpp:75)
+    const double q01 = 0.0069 * gw::exp(0.0272 * V);
+    const double q00 = -q01;
+    const double q10 = 0.0227 * gw::exp(-0.0431 * V);
+    const double q11 = -(q10 + q12);
+    const double q23 = 0.0218 * gw::exp(0.0262 * V);
+    const double q24 = 1.29e-5 * gw::exp(2.71e-6 * V);
+    const double q22 = -(q21 + q23 + q24);
+    const double q32 = 0.0009 * gw::exp(-0.0269 * V);
+    const double q34 = 0.0622 * gw::exp(0.0120 * V);
+    const double q33 = -(q32 + q34);
+    const double q43 = 0.0059 * gw::exp(-0.0443 * V);
+    const double q42 = q32 * q43 * q24 / (q23 * q34);
+    const double q44 = -(q42 + q43);
+    d[0] = dt * (q10 * x[1] + q00 * x[0]);
+    d[1] = dt * (q01 * x[0] + q21 * x[2] + q11 * x[1]);
+    d[2] = dt * (q12 * x[1] + q32 * x[3] + q42 * x[4] + q22 * x[2]);
+    d[3] = dt * (q23 * x[2] + q43 * x[4] + q33 * x[3]);
+    d[4] = dt * (q24 * x[2] + q34 * x[3] + q44 * x[4]);
+}
+
+// One Kv chain (Kv1.4 or Kv4.3): generator entries of update_QKv (GW.tpp:64-123)
+// and the derivatives of update_Kv_derivatives (GW.tpp:135-160), in registers.
+GW_HD void kv_chain(double aa, double ai, double ba, double bi, double f1, double f2, double f3, double f4,
+                    double b1, double b2, double b3, double b4, const double y[10], double dt, double d[10])
+{
+    const double q01 = 4 * aa, q05 = bi, q00 = -(q01 + q05);
+    const double q10 = ba, q12 = 3 * aa, q16 = f1 * bi, q11 = -(q10 + q12 + q16);
+    const double q21 = 2 * ba, q23 = 2 * aa, q27 = f2 * bi, q22 = -(q21 + q23 + q27);
+    const double q32 = 3 * ba, q34 = aa, q38 = f3 * bi, q33 = -(q32 + q34 + q38);
+    const double q43 = 4 * ba, q49 = f4 * bi, q44 = -(q43 + q49);
+    const double q56 = 4 * aa * b1, q50 = ai, q55 = -(q56 + q50);
+    const double q65 = ba / f1, q67 = 3 * aa * b2 / b1, q61 = ai / b1, q66 = -(q65 + q67 + q61);
+    const double q76 = 2 * ba * f1 / f2, q78 = 2 * aa * b3 / b2, q72 = ai / b2, q77 = -(q76 + q78 + q72);
+    const double q87 = 3 * ba * f2 / f3, q89 = aa * b4 / b3, q83 = ai / b3, q88 = -(q87 + q89 + q83);
+    const double q98 = 4 * ba * f3 / f4, q94 = ai / b4, q99 = -(q98 + q94);
+    d[0] = dt * (q10 * y[1] + q50 * y[5] + q00 * y[0]);
+    d[1] = dt * (q01 * y[0] + q21 * y[2] + q61 * y[6] + q11 * y[1]);
+    d[2] = dt * (q12 * y[1] + q32 * y[3] + q72 * y[7] + q22 * y[2]);
+    d[3] = dt * (q23 * y[2] + q43 * y[4] + q83 * y[8] + q33 * y[3]);
+    d[4] = dt * (q34 * y[3] + q94 * y[9] + q44 * y[4]);
+    d[5] = dt * (q65 * y[6] + q05 * y[0] + q55 * y[5]);
+    d[6] = dt * (q56 * y[5] + q76 * y[7] + q16 * y[1] + q66 * y[6]);
+    d[7] = dt * (q67 * y[6] + q87 * y[8] + q27 * y[2] + q77 * y[7]);
+    d[8] = dt * (q78 * y[7] + q98 * y[9] + q38 * y[3] + q88 * y[8]);
+    d[9] = dt * (q89 * y[8] + q49 * y[4] + q99 * y[9]);
 }
 
 // Currents of one step, in the order of GW::Currents (GW.hpp:25-42).
@@ -238,7 +233,7 @@ struct Currents {
 
 GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT,
                              double expmVF_RT, double dt, double Istim, const FluxSums &S, gw_globals_t &dg,
-                             Currents &cur, const QMatrices &Q)
+                             Currents &cur)
 {
     const double V = g.V;
     bool ok = true;
@@ -319,38 +314,21 @@ GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw
     dg.h = dt * (alpha_h(V) * (1.0 - g.h) - beta_h(V) * g.h);
     dg.j = dt * (alpha_j(V) * (1.0 - g.j) - beta_j(V) * g.j);
     dg.xKs = dt * (XKsinf(V) - g.xKs) * tauXKs_inv(V);
-    // GW.tpp:126-133
+    // GW.tpp:25-44, 64-160: the IKr and Kv chains
+    kr_chain(p, V, g.Kr, dt, dg.Kr);
     {
-        const double *q = Q.Kr;
-        const double *x = g.Kr;
-#define KR(i, j) q[(i) * 5 + (j)]
-        dg.Kr[0] = dt * (KR(1, 0) * x[1] + KR(0, 0) * x[0]);
-        dg.Kr[1] = dt * (KR(0, 1) * x[0] + KR(2, 1) * x[2] + KR(1, 1) * x[1]);
-        dg.Kr[2] = dt * (KR(1, 2) * x[1] + KR(3, 2) * x[3] + KR(4, 2) * x[4] + KR(2, 2) * x[2]);
-        dg.Kr[3] = dt * (KR(2, 3) * x[2] + KR(4, 3) * x[4] + KR(3, 3) * x[3]);
-        dg.Kr[4] = dt * (KR(2, 4) * x[2] + KR(3, 4) * x[3] + KR(4, 4) * x[4]);
-#undef KR
-    }
-    // GW.tpp:135-160
-    const double *qs[2] = {Q.Kv14, Q.Kv43};
-    const double *ys[2] = {g.Kv14, g.Kv43};
-    double *ds[2] = {dg.Kv14, dg.Kv43};
-    for (int ci = 0; ci < 2; ++ci) {
-        const double *q = qs[ci];
-        const double *y = ys[ci];
-        double *d = ds[ci];
-#define A(i, j) q[(i) * 10 + (j)]
-        d[0] = dt * (A(1, 0) * y[1] + A(5, 0) * y[5] + A(0, 0) * y[0]);
-        d[1] = dt * (A(0, 1) * y[0] + A(2, 1) * y[2] + A(6, 1) * y[6] + A(1, 1) * y[1]);
-        d[2] = dt * (A(1, 2) * y[1] + A(3, 2) * y[3] + A(7, 2) * y[7] + A(2, 2) * y[2]);
-        d[3] = dt * (A(2, 3) * y[2] + A(4, 3) * y[4] + A(8, 3) * y[8] + A(3, 3) * y[3]);
-        d[4] = dt * (A(3, 4) * y[3] + A(9, 4) * y[9] + A(4, 4) * y[4]);
-        d[5] = dt * (A(6, 5) * y[6] + A(0, 5) * y[0] + A(5, 5) * y[5]);
-        d[6] = dt * (A(5, 6) * y[5] + A(7, 6) * y[7] + A(1, 6) * y[1] + A(6, 6) * y[6]);
-        d[7] = dt * (A(6, 7) * y[6] + A(8, 7) * y[8] + A(2, 7) * y[2] + A(7, 7) * y[7]);
-        d[8] = dt * (A(7, 8) * y[7] + A(9, 8) * y[9] + A(3, 8) * y[3] + A(8, 8) * y[8]);
-        d[9] = dt * (A(8, 9) * y[8] + A(4, 9) * y[4] + A(9, 9) * y[9]);
-#undef A
+        const double aa14 = p.alphaa0Kv14 * gw::exp(p.aaKv14 * V);
+        const double aa43 = p.alphaa0Kv43 * gw::exp(p.aaKv43 * V);
+        const double ai14 = p.alphai0Kv14 * gw::exp(-p.aiKv14 * V);
+        const double ai43 = p.alphai0Kv43 * gw::exp(-p.aiKv43 * V);
+        const double ba14 = p.betaa0Kv14 * gw::exp(-p.baKv14 * V);
+        const double ba43 = p.betaa0Kv43 * gw::exp(-p.baKv43 * V);
+        const double bi14 = p.betai0Kv14 * gw::exp(p.biKv14 * V);
+        const double bi43 = p.betai0Kv43 * gw::exp(p.biKv43 * V);
+        kv_chain(aa14, ai14, ba14, bi14, p.f1Kv14, p.f2Kv14, p.f3Kv14, p.f4Kv14, p.b1Kv14, p.b2Kv14, p.b3Kv14,
+                 p.b4Kv14, g.Kv14, dt, dg.Kv14);
+        kv_chain(aa43, ai43, ba43, bi43, p.f1Kv43, p.f2Kv43, p.f3Kv43, p.f4Kv43, p.b1Kv43, p.b2Kv43, p.b3Kv43,
+                 p.b4Kv43, g.Kv43, dt, dg.Kv43);
     }
     return GW_OK;
 }

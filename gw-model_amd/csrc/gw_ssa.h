@@ -94,10 +94,11 @@ GW_HD void ryr_rates(const SsaParams &P, uint64_t w, double ca, F &&f)
 
 // Subunit total in the reference accumulation order (SSA.tpp:48-56):
 // 0 + LCC sum, + inactivation, + RyR[0..11], + ClCh.
-GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, uint32_t chan, uint64_t w, int j, double ca)
+GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, const LccTable &T, uint32_t chan, uint64_t w,
+                           int j, double ca)
 {
     const int L = chan_lcc(chan, j);
-    double s = K.lccP[L] + K.lccQ[L] * ca;
+    double s = T.P[L] + T.Q[L] * ca;
     s += chan_y(chan, j) ? K.inact1 : K.inact0;
     ryr_rates(P, w, ca, [&](int, double r) { s += r; });
     s += chan_cl(chan, j) ? P.kbClCh : P.kfClCh * ca;
@@ -203,7 +204,8 @@ constexpr uint32_t kMaxItersPerStep = 1u << 22;
 
 // One global step of SSA_single_CRU (SSA.tpp:286-310) on a register-resident CRU.
 template <int PRNG, bool DIGEST>
-GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double dt, Digest &dig)
+GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, const LccTable &T, double dt, Digest &dig,
+                      uint32_t step_lo)
 {
     CruOut out;
     out.iters = 0;
@@ -227,7 +229,7 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
         Jtr = P.rtr * (K.CaNSR - c.jsr);
         double s[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, c.chan, c.ryr[j], j, old[j]);
+        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, c.chan, c.ryr[j], j, old[j]);
         const double total = s[0] + s[1] + s[2] + s[3];
         const double jump = -gw::log(draw<PRNG>(c)) / total;
         const bool last = !(t + jump < dt);
@@ -255,7 +257,7 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
                     w = ryr_set(ryr_set(w, 4, 0), 5, 0);
                 }
                 c.ryr[j] = w;
-                if (DIGEST) digest_event(dig, K.step_lo, j, 4, ryr_get(w, 4));
+                if (DIGEST) digest_event(dig, step_lo, j, 4, ryr_get(w, 4));
             } else if (old[j] > 0.03685 && tmp <= 0.03685) {
                 // sample_RyR34 (SSA.tpp:73-84)
                 uint64_t w = c.ryr[j];
@@ -270,7 +272,7 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
                     w = ryr_set(ryr_set(w, 2, 0), 3, 0);
                 }
                 c.ryr[j] = w;
-                if (DIGEST) digest_event(dig, K.step_lo, j, 5, ryr_get(w, 2));
+                if (DIGEST) digest_event(dig, step_lo, j, 5, ryr_get(w, 2));
             }
             c.ca[j] = tmp;
         }
@@ -302,7 +304,7 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
         const int L = chan_lcc(c.chan, j);
         const int y = chan_y(c.chan, j);
         const int cl = chan_cl(c.chan, j);
-        const double sum_LCC = K.lccP[L] + K.lccQ[L] * caj;
+        const double sum_LCC = T.P[L] + T.Q[L] * caj;
         const double inact = y ? K.inact1 : K.inact0;
         const double clr = cl ? P.kbClCh : P.kfClCh * caj;
         const double sum_RyR = sj - (sum_LCC + inact + clr);
@@ -312,8 +314,8 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
         if (u < sum_LCC) {
             // sample_LCC (SSA.tpp:103-197)
             // the scan over (r0, r1, r2) never compares after adding r2, so r2 is not needed
-            const double r0 = K.lccR0[L];
-            const double r1 = (L == 1) ? K.lccQ[L] * caj : K.lccR1[L];
+            const double r0 = T.R0[L];
+            const double r1 = (L == 1) ? T.Q[L] * caj : T.R1[L];
             const double uu = draw<PRNG>(c) * sum_LCC;
             double cum = r0;
             int tr = 0;
@@ -325,13 +327,13 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
             else if (L == 6 || L == 12)
                 jl_new = 0.0;
             c.chan = chan_set_lcc(c.chan, j, Ln);
-            if (DIGEST) digest_event(dig, K.step_lo, j, 0, Ln);
+            if (DIGEST) digest_event(dig, step_lo, j, 0, Ln);
         } else if (u < (sum_LCC + inact)) {
             const int yn = 1 - y;
             c.chan = chan_flip_y(c.chan, j);
             if (yn == 0) jl_new = 0.0;
             else if (L == 6 || L == 12) jl_new = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca_new);
-            if (DIGEST) digest_event(dig, K.step_lo, j, 1, yn);
+            if (DIGEST) digest_event(dig, step_lo, j, 1, yn);
         } else if (u < (sum_LCC + inact + sum_RyR)) {
             // sample_RyR (SSA.tpp:200-259)
             const double uu = draw<PRNG>(c) * sum_RyR;
@@ -358,10 +360,10 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, double 
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (q == j) { c.ryr[q] = w; open[q] = oj; Jrel[q] = jr; }
-            if (DIGEST) digest_event(dig, K.step_lo, j, 2, tr);
+            if (DIGEST) digest_event(dig, step_lo, j, 2, tr);
         } else {
             c.chan = chan_flip_cl(c.chan, j);
-            if (DIGEST) digest_event(dig, K.step_lo, j, 3, 1 - cl);
+            if (DIGEST) digest_event(dig, step_lo, j, 3, 1 - cl);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
