@@ -32,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "gw-model_amd")
 
 F_ITER = 330            # FP64 flops of one SSA loop iteration (count in DESIGN.md)
-STATE_BYTES = 280       # bytes read + written per CRU-step by the FP64 SSA kernel (layout in DESIGN.md)
+STATE_BYTES = 272       # bytes read + written per CRU-step by the FP64 SSA kernel: 136 in, 136 out
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 

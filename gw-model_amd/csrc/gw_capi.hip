@@ -1,18 +1,19 @@
 // gw_capi.hip -- kernels of the MI355X Greenstein-Winslow integrator and the
 // C-ABI of include/gw_capi.h.
 //
-// Device layout: every CRU of every cell owns one slot of structure-of-arrays
-// state (CaSS[4][ntot], CaJSR, JLCC[4][ntot], packed channel word, four packed
-// RyR count words, four engine words). Each cell's slots are padded to whole
-// 256-thread blocks, so a block belongs to exactly one cell and reads that
-// cell's constants once.
+// Device layout: every CRU of every cell owns one slot. Per-subunit words are
+// stored [slot][4] (CaSS, JLCC, the packed subunit word holding the six RyR
+// counts, LCC state, y gate and ClCh); CaJSR is [slot]; the engine state is
+// [slot][4] words and the replay digest [slot][2]. A CRU is advanced by a quad
+// of lanes (lane j owns subunit j), so one 256-thread block holds 64 CRUs; each
+// cell's slots are padded to whole blocks, so a block belongs to one cell.
 //
-// One Euler step (GW_model::euler_step, src/includes/GW.tpp:236-272) is one
-// launch of k_ssa_step: every block advances its CRUs through the SSA with the
-// cell's frozen constants, reduces the flux sums exactly (128-bit fixed point)
-// into a block partial, and the last block of the cell to finish (an agent-scope
-// acq_rel counter) forms the cell's sums, applies globals += dGlobals and
-// prepares the next step: derivatives at the new state and the SSA constants.
+// One Euler step (GW_model::euler_step, src/includes/GW.tpp:236-272) is two
+// launches: k_ssa advances every CRU through SSA(dt) with the cell's frozen
+// constants and reduces the flux sums exactly (128-bit fixed point) into one
+// partial per block; k_cell_close (one wave per cell) sums the partials,
+// applies globals += dGlobals and prepares the next step: derivatives at the
+// new state and the SSA constants.
 #include "gw_kernels.h"
 
 #include <algorithm>
@@ -131,6 +132,12 @@ __device__ void sum_block_partials(const Partial *parts, int64_t first, int n, P
     }
 }
 
+__device__ __forceinline__ void partial_zero(Partial &m)
+{
+    fx_zero(m.jlcc); fx_zero(m.jxfer); fx_zero(m.jtr); fx_zero(m.jrel);
+    m.clch = 0; m.open = 0; m.iters = 0; m.err = 0;
+}
+
 __device__ void partial_add(Partial &a, const Partial &b)
 {
     fx_add(a.jlcc, b.jlcc);
@@ -158,56 +165,54 @@ __device__ void block_reduce(Partial *red, Partial mine)
 // ===========================================================================
 // kernels
 // ===========================================================================
+// Thread t of an SSA-shaped block handles subunit (t & 3) of CRU (t >> 2).
 
-// Seed the per-CRU engines and replay digests; zero the padding slots.
+// Seed the per-CRU engines and replay digests; give padding slots an inert state.
 __global__ void k_seed(KArgs a)
 {
     const int b = blockIdx.x;
     const int cell = a.block_cell[b];
     const DevCell &C = a.cells[cell];
-    const int64_t i = (b - C.blk0) * kBlock + threadIdx.x;
+    const int j = threadIdx.x & 3;
+    const int64_t i = (b - C.blk0) * kCruPerBlock + (threadIdx.x >> 2);
     const int64_t slot = C.slot0 + i;
-    uint64_t s[4];
-    rng_seed(a.prng, s, stream_key(C.seed, i));
-    for (int k = 0; k < 4; ++k) a.rng[k * a.ntot + slot] = s[k];
-    if (a.dig) {
-        a.dig[slot] = 0;
-        a.dig[a.ntot + slot] = 0xcbf29ce484222325ULL;
-    }
-    if (i >= C.n_cru) {  // padding: a valid, inert state
-        for (int j = 0; j < 4; ++j) {
-            a.CaSS[j * a.ntot + slot] = 0.0;
-            a.JLCC[j * a.ntot + slot] = 0.0;
-            a.ryr[j * a.ntot + slot] = 0;
+    if (j == 0) {
+        uint64_t s[4];
+        rng_seed(a.prng, s, stream_key(C.seed, i));
+        for (int k = 0; k < 4; ++k) a.rng[slot * 4 + k] = s[k];
+        if (a.dig) {
+            a.dig[slot * 2] = 0;
+            a.dig[slot * 2 + 1] = 0xcbf29ce484222325ULL;
         }
-        a.CaJSR[slot] = 0.0;
-        a.chan[slot] = 0x1111u;
+        if (i >= C.n_cru) a.CaJSR[slot] = 0.0;
+    }
+    if (i >= C.n_cru) {
+        a.CaSS[slot * 4 + j] = 0.0;
+        a.JLCC[slot * 4 + j] = 0.0;
+        a.sub[slot * 4 + j] = sub_set_lcc(0, 1);
     }
 }
 
-// Rest-state sampler (initialise_LCC/_LCC_i/_RyR/_ClCh, src/includes/GW_utils.tpp:5-64,
-// and the CRUState constants, GW_utils.tpp:212-213) on a per-CRU stream keyed
-// by the complement of the CRU's SSA stream key.
+// Rest-state sampler (src/includes/GW_utils.tpp:5-64) of every CRU.
 template <int PRNG>
 __global__ void k_sample_rest(KArgs a)
 {
     const int b = blockIdx.x;
     const int cell = a.block_cell[b];
     const DevCell &C = a.cells[cell];
-    const int64_t i = (b - C.blk0) * kBlock + threadIdx.x;
+    if ((threadIdx.x & 3) != 0) return;
+    const int64_t i = (b - C.blk0) * kCruPerBlock + (threadIdx.x >> 2);
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
     double ca[4], jsr;
-    uint32_t chan;
-    uint64_t ryr[4];
-    sample_rest_cru<PRNG>(C.seed, i, ca, jsr, chan, ryr);
+    uint64_t sub[4];
+    sample_rest_cru<PRNG>(C.seed, i, ca, jsr, sub);
     for (int j = 0; j < 4; ++j) {
-        a.CaSS[j * a.ntot + slot] = ca[j];
-        a.JLCC[j * a.ntot + slot] = 0.0;
-        a.ryr[j * a.ntot + slot] = ryr[j];
+        a.CaSS[slot * 4 + j] = ca[j];
+        a.JLCC[slot * 4 + j] = 0.0;
+        a.sub[slot * 4 + j] = sub[j];
     }
     a.CaJSR[slot] = jsr;
-    a.chan[slot] = chan;
 }
 
 // Step-0 preparation of every cell (one block per cell): the flux sums the
@@ -221,22 +226,20 @@ __global__ void k_init_cells(KArgs a)
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
     Partial mine;
-    fx_zero(mine.jlcc); fx_zero(mine.jxfer); fx_zero(mine.jtr); fx_zero(mine.jrel);
-    mine.clch = 0; mine.open = 0; mine.iters = 0; mine.err = 0;
+    partial_zero(mine);
     const double X = C.p.rxfer * (1.45370e-4 - 1.45273e-4);
     const double X4 = X + X + X + X;
     const double Y = C.p.rtr * (0.908882 - 0.908408);
     for (int64_t i = threadIdx.x; i < C.n_cru; i += kBlock) {
         const int64_t slot = C.slot0 + i;
         double jl[4];
-        for (int j = 0; j < 4; ++j) jl[j] = a.JLCC[j * a.ntot + slot];
+        for (int j = 0; j < 4; ++j) jl[j] = a.JLCC[slot * 4 + j];
         mine.err |= fx_add(mine.jlcc, jl[0] + jl[1] + jl[2] + jl[3]);
         mine.err |= fx_add(mine.jxfer, X4);
         mine.err |= fx_add(mine.jtr, Y);
-        const uint32_t ch = a.chan[slot];
         for (int j = 0; j < 4; ++j) {
-            mine.clch += chan_cl(ch, j);
-            const uint64_t w = a.ryr[j * a.ntot + slot];
+            const uint64_t w = a.sub[slot * 4 + j];
+            mine.clch += sub_cl(w);
             mine.open += ryr_get(w, 2) + ryr_get(w, 3);
         }
     }
@@ -255,47 +258,16 @@ __global__ void k_init_cells(KArgs a)
     }
 }
 
-__device__ __forceinline__ void load_cru(const KArgs &a, int64_t slot, Cru64 &c)
+// Copy this step's LCC rate table into LDS (it is read with a per-lane index;
+// the cell's other constants are wave-uniform and are read as scalars).
+__device__ __forceinline__ void load_lcc_table(const CellDyn &D, LccTable &T)
 {
-    const int64_t N = a.ntot;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        c.ca[j] = a.CaSS[j * N + slot];
-        c.jl[j] = a.JLCC[j * N + slot];
-        c.ryr[j] = a.ryr[j * N + slot];
-    }
-    c.jsr = a.CaJSR[slot];
-    c.chan = a.chan[slot];
-    c.r0 = a.rng[slot];
-    c.r1 = a.rng[N + slot];
-    c.r2 = a.rng[2 * N + slot];
-    c.r3 = a.rng[3 * N + slot];
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&D.k.lcc);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(&T);
+    for (int w = threadIdx.x; w < static_cast<int>(sizeof(LccTable) / 8); w += blockDim.x) dst[w] = src[w];
 }
 
-__device__ __forceinline__ void store_cru(const KArgs &a, int64_t slot, const Cru64 &c)
-{
-    const int64_t N = a.ntot;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        a.CaSS[j * N + slot] = c.ca[j];
-        a.JLCC[j * N + slot] = c.jl[j];
-        a.ryr[j * N + slot] = c.ryr[j];
-    }
-    a.CaJSR[slot] = c.jsr;
-    a.chan[slot] = c.chan;
-    a.rng[slot] = c.r0;
-    a.rng[N + slot] = c.r1;
-    a.rng[2 * N + slot] = c.r2;
-    a.rng[3 * N + slot] = c.r3;
-}
-
-__device__ __forceinline__ void partial_zero(Partial &m)
-{
-    fx_zero(m.jlcc); fx_zero(m.jxfer); fx_zero(m.jtr); fx_zero(m.jrel);
-    m.clch = 0; m.open = 0; m.iters = 0; m.err = 0;
-}
-
-__device__ __forceinline__ void partial_from(Partial &m, const CruOut &o, uint64_t iters)
+__device__ __forceinline__ void partial_from(Partial &m, const QuadOut &o, uint64_t iters)
 {
     m.err |= fx_add(m.jlcc, o.JLCC);
     m.err |= fx_add(m.jxfer, o.Jxfer);
@@ -307,24 +279,18 @@ __device__ __forceinline__ void partial_from(Partial &m, const CruOut &o, uint64
     m.err |= o.err ? 4 : 0;
 }
 
-// Copy this step's LCC rate table into LDS (it is read with a per-lane index;
-// the cell's other constants are wave-uniform and are read as scalars).
-__device__ __forceinline__ void load_lcc_table(const CellDyn &D, LccTable &T)
-{
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&D.k.lcc);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(&T);
-    for (int w = threadIdx.x; w < static_cast<int>(sizeof(LccTable) / 8); w += blockDim.x) dst[w] = src[w];
-}
-
-// SSA(dt) of one step (GW_model::SSA, src/includes/GW.tpp:214-234) for the
-// CRUs of this block; the block's flux partial goes to partials[block].
+// SSA(dt) of `nsteps` steps (GW_model::SSA, src/includes/GW.tpp:214-234) for
+// the CRUs of this block with the cell's current constants; nsteps > 1 only in
+// frozen-globals replay. The block's flux partial of the last step goes to
+// partials[block]; only the quad leaders contribute.
 template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock, 4) void k_ssa_step(KArgs a)
+__global__ __launch_bounds__(kBlock, 4) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
 {
     __shared__ LccTable T;
     __shared__ Partial red[kBlock];
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
+    const int j = tid & 3;
     const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
     const DevCell &C = a.cells[cell];
     const CellDyn &D = a.dyn[cell];
@@ -332,26 +298,48 @@ __global__ __launch_bounds__(kBlock, 4) void k_ssa_step(KArgs a)
     const StepConsts &kk = D.k;
     load_lcc_table(D, T);
     __syncthreads();
-    const int64_t i = (b - C.blk0) * kBlock + tid;
+    const int64_t i = (b - C.blk0) * kCruPerBlock + (tid >> 2);
     const bool active = (i < C.n_cru) && (D.status == 0);
     Partial mine;
     partial_zero(mine);
     if (active) {
         const int64_t slot = C.slot0 + i;
-        Cru64 c;
-        load_cru(a, slot, c);
+        double ca = a.CaSS[slot * 4 + j];
+        double jl = a.JLCC[slot * 4 + j];
+        uint64_t w = a.sub[slot * 4 + j];
+        CruQ q;
+        q.jsr = a.CaJSR[slot];
+        q.r0 = a.rng[slot * 4];
+        q.r1 = a.rng[slot * 4 + 1];
+        q.r2 = a.rng[slot * 4 + 2];
+        q.r3 = a.rng[slot * 4 + 3];
         Digest dg{0, 0};
         if (DIGEST) {
-            dg.count = a.dig[slot];
-            dg.hash = a.dig[a.ntot + slot];
+            dg.count = a.dig[slot * 2];
+            dg.hash = a.dig[slot * 2 + 1];
         }
-        const CruOut o = ssa_step<PRNG, DIGEST>(c, sp, kk, T, a.dt, dg, kk.step_lo);
-        store_cru(a, slot, c);
-        if (DIGEST) {
-            a.dig[slot] = dg.count;
-            a.dig[a.ntot + slot] = dg.hash;
+        QuadOut o{};
+        uint64_t iters = 0;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            o = ssa_step_quad<PRNG, DIGEST>(ca, jl, w, q, sp, kk, T, a.dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff), j);
+            iters += o.iters;
+            if (o.err) break;
         }
-        partial_from(mine, o, o.iters);
+        a.CaSS[slot * 4 + j] = ca;
+        a.JLCC[slot * 4 + j] = jl;
+        a.sub[slot * 4 + j] = w;
+        if (j == 0) {
+            a.CaJSR[slot] = q.jsr;
+            a.rng[slot * 4] = q.r0;
+            a.rng[slot * 4 + 1] = q.r1;
+            a.rng[slot * 4 + 2] = q.r2;
+            a.rng[slot * 4 + 3] = q.r3;
+            if (DIGEST) {
+                a.dig[slot * 2] = dg.count;
+                a.dig[slot * 2 + 1] = dg.hash;
+            }
+            partial_from(mine, o, iters);
+        }
     }
     block_reduce<kBlock>(red, mine);
     if (tid == 0) a.partials[b] = red[0];
@@ -404,54 +392,6 @@ __global__ __launch_bounds__(kWave) void k_cell_close(KArgs a)
     }
 }
 
-// Frozen-globals replay: every lane advances its CRU through `nsteps` steps
-// with the cell's constant SSA constants (globals held fixed), so only the
-// SSA runs. The flux sums of the final step are reduced as in k_ssa_step.
-template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock, 4) void k_ssa_frozen(KArgs a, int64_t step0, int64_t nsteps)
-{
-    __shared__ LccTable T;
-    __shared__ Partial red[kBlock];
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
-    const DevCell &C = a.cells[cell];
-    const CellDyn &D = a.dyn[cell];
-    const SsaParams &sp = C.sp;
-    const StepConsts &kk = D.k;
-    load_lcc_table(D, T);
-    __syncthreads();
-    const int64_t i = (b - C.blk0) * kBlock + tid;
-    const bool active = (i < C.n_cru) && (D.status == 0);
-    Partial mine;
-    partial_zero(mine);
-    if (active) {
-        const int64_t slot = C.slot0 + i;
-        Cru64 c;
-        load_cru(a, slot, c);
-        Digest dg{0, 0};
-        if (DIGEST) {
-            dg.count = a.dig[slot];
-            dg.hash = a.dig[a.ntot + slot];
-        }
-        CruOut o{};
-        uint64_t iters = 0;
-        for (int64_t s = 0; s < nsteps; ++s) {
-            o = ssa_step<PRNG, DIGEST>(c, sp, kk, T, a.dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff));
-            iters += o.iters;
-            if (o.err) break;
-        }
-        store_cru(a, slot, c);
-        if (DIGEST) {
-            a.dig[slot] = dg.count;
-            a.dig[a.ntot + slot] = dg.hash;
-        }
-        partial_from(mine, o, iters);
-    }
-    block_reduce<kBlock>(red, mine);
-    if (tid == 0) a.partials[b] = red[0];
-}
-
 // Close a frozen replay of nsteps: sums of the last step, clock advanced as
 // t += dt per step (src/pyGW.hpp:116).
 __global__ void k_frozen_close(KArgs a, int64_t nsteps)
@@ -471,7 +411,7 @@ __global__ void k_frozen_close(KArgs a, int64_t nsteps)
     D.k.step_lo = static_cast<uint32_t>(D.step & 0xffff);
 }
 
-// SoA <-> row-major host layout of one cell (GWCRUState, src/pyGW.cpp:78-87).
+// Packed device state <-> row-major host layout of one cell (GWCRUState, src/pyGW.cpp:78-87).
 struct StateView {
     double *CaSS, *CaJSR, *JLCC;
     int32_t *LCC, *LCCi, *RyR, *ClCh;
@@ -483,19 +423,14 @@ __global__ void k_put_state(KArgs a, int cell, StateView v, int has_jlcc)
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
-    uint32_t chan = 0;
     for (int j = 0; j < 4; ++j) {
-        a.CaSS[j * a.ntot + slot] = v.CaSS[i * 4 + j];
-        a.JLCC[j * a.ntot + slot] = has_jlcc ? v.JLCC[i * 4 + j] : 0.0;
-        chan = chan_set_lcc(chan, j, v.LCC[i * 4 + j]);
-        if (v.LCCi[i * 4 + j]) chan |= 1u << (16 + j);
-        if (v.ClCh[i * 4 + j]) chan |= 1u << (20 + j);
-        uint64_t w = 0;
-        for (int k = 0; k < 6; ++k) w = ryr_set(w, k, v.RyR[(i * 4 + j) * 6 + k]);
-        a.ryr[j * a.ntot + slot] = w;
+        a.CaSS[slot * 4 + j] = v.CaSS[i * 4 + j];
+        a.JLCC[slot * 4 + j] = has_jlcc ? v.JLCC[i * 4 + j] : 0.0;
+        int n6[6];
+        for (int k = 0; k < 6; ++k) n6[k] = v.RyR[(i * 4 + j) * 6 + k];
+        a.sub[slot * 4 + j] = sub_pack(v.LCC[i * 4 + j], v.LCCi[i * 4 + j], v.ClCh[i * 4 + j], n6);
     }
     a.CaJSR[slot] = v.CaJSR[i];
-    a.chan[slot] = chan;
 }
 
 __global__ void k_get_state(KArgs a, int cell, StateView v)
@@ -504,14 +439,13 @@ __global__ void k_get_state(KArgs a, int cell, StateView v)
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
-    const uint32_t chan = a.chan[slot];
     for (int j = 0; j < 4; ++j) {
-        v.CaSS[i * 4 + j] = a.CaSS[j * a.ntot + slot];
-        v.JLCC[i * 4 + j] = a.JLCC[j * a.ntot + slot];
-        v.LCC[i * 4 + j] = chan_lcc(chan, j);
-        v.LCCi[i * 4 + j] = chan_y(chan, j);
-        v.ClCh[i * 4 + j] = chan_cl(chan, j);
-        const uint64_t w = a.ryr[j * a.ntot + slot];
+        v.CaSS[i * 4 + j] = a.CaSS[slot * 4 + j];
+        v.JLCC[i * 4 + j] = a.JLCC[slot * 4 + j];
+        const uint64_t w = a.sub[slot * 4 + j];
+        v.LCC[i * 4 + j] = sub_lcc(w);
+        v.LCCi[i * 4 + j] = sub_y(w);
+        v.ClCh[i * 4 + j] = sub_cl(w);
         for (int k = 0; k < 6; ++k) v.RyR[(i * 4 + j) * 6 + k] = ryr_get(w, k);
     }
     v.CaJSR[i] = a.CaJSR[slot];
@@ -528,7 +462,6 @@ __global__ void k_reset_obs(CellDyn *dyn, int ncells)
     o.v_prev = V; o.apd_v_rest = V; o.apd_v_peak = V; o.apd_t_up = -1.0;
     o.apd90_last = -1.0; o.apd90_prev = -1.0; o.alt_sum = 0; o.alt_count = 0; o.beats_done = 0;
 }
-
 // ===========================================================================
 // host side
 // ===========================================================================
@@ -653,9 +586,9 @@ template <int PRNG>
 void launch_frozen(gw_batch *B, int64_t step0, int64_t n)
 {
     if (B->digest)
-        hipLaunchKernelGGL((k_ssa_frozen<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+        hipLaunchKernelGGL((k_ssa<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
     else
-        hipLaunchKernelGGL((k_ssa_frozen<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+        hipLaunchKernelGGL((k_ssa<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
     hipLaunchKernelGGL(k_frozen_close, dim3((B->ncells + 63) / 64), dim3(64), 0, B->ctx->stream, B->a, n);
 }
 
@@ -672,25 +605,25 @@ void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
 }
 
 template <int PRNG>
-void launch_step(gw_batch *B)
+void launch_step(gw_batch *B, int64_t step)
 {
     if (B->digest)
-        hipLaunchKernelGGL((k_ssa_step<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
+        hipLaunchKernelGGL((k_ssa<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step, int64_t(1));
     else
-        hipLaunchKernelGGL((k_ssa_step<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a);
+        hipLaunchKernelGGL((k_ssa<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step, int64_t(1));
     hipLaunchKernelGGL(k_cell_close, dim3(B->ncells), dim3(kWave), 0, B->ctx->stream, B->a);
 }
 
-void launch_steps(gw_batch *B, int64_t n)
+void launch_steps(gw_batch *B, int64_t step0, int64_t n)
 {
-    for (int64_t s = 0; s < n; ++s) {
+    for (int64_t s = step0; s < step0 + n; ++s) {
         switch (B->prng) {
-        case 0: launch_step<0>(B); break;
-        case 1: launch_step<1>(B); break;
-        case 2: launch_step<2>(B); break;
-        case 3: launch_step<3>(B); break;
-        case 4: launch_step<4>(B); break;
-        default: launch_step<5>(B); break;
+        case 0: launch_step<0>(B, s); break;
+        case 1: launch_step<1>(B, s); break;
+        case 2: launch_step<2>(B, s); break;
+        case 3: launch_step<3>(B, s); break;
+        case 4: launch_step<4>(B, s); break;
+        default: launch_step<5>(B, s); break;
         }
     }
 }
@@ -789,17 +722,16 @@ int gw_rest_state(int32_t n_cru, uint64_t seed, int32_t prng_id, gw_cru_state_t 
     if (n_cru <= 0 || !out || prng_id < 0 || prng_id >= GW_PRNG_COUNT) return GW_E_INVALID_ARG;
     for (int64_t i = 0; i < n_cru; ++i) {
         double ca[4], jsr;
-        uint32_t chan;
-        uint64_t ryr[4];
-        sample_rest_cru_any(prng_id, seed, i, ca, jsr, chan, ryr);
+        uint64_t sub[4];
+        sample_rest_cru_any(prng_id, seed, i, ca, jsr, sub);
         for (int j = 0; j < 4; ++j) {
             if (out->CaSS) out->CaSS[i * 4 + j] = ca[j];
             if (out->JLCC) out->JLCC[i * 4 + j] = 0.0;
-            if (out->LCC) out->LCC[i * 4 + j] = chan_lcc(chan, j);
-            if (out->LCC_inactivation) out->LCC_inactivation[i * 4 + j] = chan_y(chan, j);
-            if (out->ClCh) out->ClCh[i * 4 + j] = chan_cl(chan, j);
+            if (out->LCC) out->LCC[i * 4 + j] = sub_lcc(sub[j]);
+            if (out->LCC_inactivation) out->LCC_inactivation[i * 4 + j] = sub_y(sub[j]);
+            if (out->ClCh) out->ClCh[i * 4 + j] = sub_cl(sub[j]);
             if (out->RyR)
-                for (int k = 0; k < 6; ++k) out->RyR[(i * 4 + j) * 6 + k] = ryr_get(ryr[j], k);
+                for (int k = 0; k < 6; ++k) out->RyR[(i * 4 + j) * 6 + k] = ryr_get(sub[j], k);
         }
         if (out->CaJSR) out->CaJSR[i] = jsr;
     }
@@ -858,7 +790,7 @@ int gw_batch_destroy(gw_batch *B)
     live.erase(std::remove(live.begin(), live.end(), B), live.end());
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
-    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.chan, B->a.ryr, B->a.rng, B->a.dig, B->a.partials,
+    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
                     B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -923,10 +855,10 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         C.proto = d.protocol;
         C.seed = d.seed;
         C.n_cru = d.n_cru;
-        C.nblk = (d.n_cru + kBlock - 1) / kBlock;
+        C.nblk = (d.n_cru + kCruPerBlock - 1) / kCruPerBlock;
         C.slot0 = slot;
         C.blk0 = blk;
-        slot += static_cast<int64_t>(C.nblk) * kBlock;
+        slot += static_cast<int64_t>(C.nblk) * kCruPerBlock;
         blk += C.nblk;
         for (int b = 0; b < C.nblk; ++b) B->block_cell_h.push_back(c);
     }
@@ -936,8 +868,8 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     KArgs &a = B->a;
     int rc = GW_OK;
     if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
-        (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.chan, N, err, errlen)) ||
-        (rc = dmalloc(&a.ryr, 4 * N, err, errlen)) || (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
+        (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
+        (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, B->nblocks, err, errlen)) ||
         (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
@@ -1141,7 +1073,7 @@ int gw_batch_step(gw_batch *B, int64_t nsteps, char *err, size_t errlen)
     const int64_t chunk = 4096;
     for (int64_t done = 0; done < nsteps;) {
         const int64_t n = std::min(chunk, nsteps - done);
-        launch_steps(B, n);
+        launch_steps(B, B->steps_done, n);
         HIP_TRY(hipGetLastError());
         done += n;
         B->steps_done += n;
@@ -1248,9 +1180,13 @@ int gw_batch_get_event_digest(gw_batch *B, int32_t cell, uint64_t *count, uint64
     }
     const DevCell &C = B->cells_h[cell];
     HIP_TRY(hipSetDevice(B->ctx->device));
-    HIP_TRY(hipMemcpyAsync(count, B->a.dig + C.slot0, C.n_cru * sizeof(uint64_t), hipMemcpyDeviceToHost, B->ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hash, B->a.dig + B->ntot + C.slot0, C.n_cru * sizeof(uint64_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    std::vector<uint64_t> pairs(2 * static_cast<size_t>(C.n_cru));
+    HIP_TRY(hipMemcpyAsync(pairs.data(), B->a.dig + 2 * C.slot0, pairs.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, B->ctx->stream));
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    for (int64_t i = 0; i < C.n_cru; ++i) {
+        count[i] = pairs[2 * i];
+        hash[i] = pairs[2 * i + 1];
+    }
     return GW_OK;
 }
 
@@ -1290,8 +1226,9 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, int64_t *launch
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, B->ctx->stream));
-    launch_steps(B, nsteps);
+    launch_steps(B, B->steps_done, nsteps);
     HIP_TRY(hipGetLastError());
+    B->steps_done += nsteps;
     HIP_TRY(hipEventRecord(e1, B->ctx->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float f = 0.f;

@@ -5,8 +5,9 @@
 
 namespace gw {
 
-constexpr int kBlock = 256;        // threads per SSA block (one CRU per thread)
+constexpr int kBlock = 256;        // threads per SSA block
 constexpr int kWave = 64;
+constexpr int kCruPerBlock = kBlock / 4;  // one quad of lanes per CRU
 
 // Static description of one cell on the device.
 struct DevCell {
@@ -16,7 +17,7 @@ struct DevCell {
     gw_protocol_t proto;
     const double *istim;   // per-step stimulus table (GW_PROTO_TABLE) or null
     int64_t istim_len;
-    int64_t slot0;         // first CRU slot of this cell in the SoA arrays
+    int64_t slot0;         // first CRU slot of this cell in the state arrays
     int64_t blk0;          // first SSA block of this cell
     uint64_t seed;
     int32_t n_cru;
@@ -51,13 +52,12 @@ struct KArgs {
     const DevCell *cells;
     CellDyn *dyn;
     const int32_t *block_cell;
-    double *CaSS;      // [4][ntot]
+    double *CaSS;      // [ntot][4]
     double *CaJSR;     // [ntot]
-    double *JLCC;      // [4][ntot]
-    uint32_t *chan;    // [ntot]
-    uint64_t *ryr;     // [4][ntot]
-    uint64_t *rng;     // [4][ntot]
-    uint64_t *dig;     // [2][ntot] (count, hash) when digests are kept
+    double *JLCC;      // [ntot][4]
+    uint64_t *sub;     // [ntot][4] packed subunit words (RyR counts, LCC, y, ClCh)
+    uint64_t *rng;     // [ntot][4] engine words
+    uint64_t *dig;     // [ntot][2] (count, hash) when digests are kept
     Partial *partials; // [nblocks]
     double *trace;     // [cap][ncells][4]
     unsigned long long *iters_total;

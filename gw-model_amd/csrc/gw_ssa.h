@@ -1,13 +1,19 @@
-// gw_ssa.h -- per-CRU Gillespie SSA over one global step, FP64, on one lane.
+// gw_ssa.h -- per-CRU Gillespie SSA over one global step, FP64, on a quad of
+// lanes (one lane per subunit).
 //
 // Restates SSA_single_CRU and its helpers (src/includes/SSA.tpp:10-310,
-// SSA.hpp:67-71, GW_utils.tpp:118-255, GW_utils.hpp:378-391) for a GPU lane:
-//   - the CRU lives in registers: CaSS[4], CaJSR, JLCC[4], packed channel
-//     states and four packed RyR count words (six signed bytes per subunit);
-//   - only the four subunit totals are kept per iteration; the individual
-//     propensities of the chosen subunit are recomputed when an event is
-//     sampled, from the same inputs the reference's stored rate arrays were
-//     formed from (CaSS before this iteration's update, RyR counts before any
+// SSA.hpp:67-71, GW_utils.tpp:118-255, GW_utils.hpp:378-391) for gfx950:
+//   - lane j of a quad owns subunit j of one CRU: its CaSS, JLCC, Jrel and one
+//     packed 64-bit word (six signed RyR counts, the LCC state, the y gate and
+//     the ClCh state). CRU-level state (CaJSR, the xoshiro stream, the step's
+//     time, the waiting time) is replicated in the four lanes and evolves
+//     identically in each, so every lane draws the same uniforms;
+//   - the 4-ring coupling Jiss (SSA.tpp:39-42) and the ordered sums over
+//     subunits are DPP quad_perm moves; sums keep the reference order
+//     ((x0 + x1) + x2) + x3 in every lane;
+//   - the propensities of the chosen subunit are recomputed when an event is
+//     sampled, from the inputs the reference's stored rate arrays were formed
+//     from (CaSS before this iteration's update, RyR counts before any
 //     threshold re-split), so every comparison sees identical values;
 //   - all arithmetic keeps the reference's operation order (bit-exact with
 //     the CPU oracle when compiled with -ffp-contract=off).
@@ -17,23 +23,30 @@
 
 namespace gw {
 
-// ---- packed channel state ----------------------------------------------
-// chan: bits 4j..4j+3 = LCC state (1..12) of subunit j; bit 16+j = y (LCC
-// voltage-inactivation gate, 1 = available); bit 20+j = ClCh open.
-GW_HD int chan_lcc(uint32_t c, int j) { return static_cast<int>((c >> (4 * j)) & 15u); }
-GW_HD int chan_y(uint32_t c, int j) { return static_cast<int>((c >> (16 + j)) & 1u); }
-GW_HD int chan_cl(uint32_t c, int j) { return static_cast<int>((c >> (20 + j)) & 1u); }
-GW_HD uint32_t chan_set_lcc(uint32_t c, int j, int s) { return (c & ~(15u << (4 * j))) | (static_cast<uint32_t>(s) << (4 * j)); }
-GW_HD uint32_t chan_flip_y(uint32_t c, int j) { return c ^ (1u << (16 + j)); }
-GW_HD uint32_t chan_flip_cl(uint32_t c, int j) { return c ^ (1u << (20 + j)); }
-
-// ---- packed RyR counts: byte k (k = 0..5) = signed count in state k+1 ---
+// ---- packed subunit word -------------------------------------------------
+// bytes 0..5: signed count of the subunit's 5 RyRs in states 1..6;
+// bits 48..51: LCC state (1..12); bit 52: y (1 = available); bit 53: ClCh open.
 GW_HD int ryr_get(uint64_t w, int k) { return static_cast<int>(static_cast<int8_t>((w >> (8 * k)) & 0xff)); }
 GW_HD uint64_t ryr_set(uint64_t w, int k, int v)
 {
     return (w & ~(0xffULL << (8 * k))) | (static_cast<uint64_t>(static_cast<uint8_t>(static_cast<int8_t>(v))) << (8 * k));
 }
 GW_HD uint64_t ryr_add(uint64_t w, int k, int d) { return ryr_set(w, k, ryr_get(w, k) + d); }
+GW_HD int sub_lcc(uint64_t w) { return static_cast<int>((w >> 48) & 15u); }
+GW_HD int sub_y(uint64_t w) { return static_cast<int>((w >> 52) & 1u); }
+GW_HD int sub_cl(uint64_t w) { return static_cast<int>((w >> 53) & 1u); }
+GW_HD uint64_t sub_set_lcc(uint64_t w, int s) { return (w & ~(15ULL << 48)) | (static_cast<uint64_t>(s) << 48); }
+GW_HD uint64_t sub_flip_y(uint64_t w) { return w ^ (1ULL << 52); }
+GW_HD uint64_t sub_flip_cl(uint64_t w) { return w ^ (1ULL << 53); }
+GW_HD uint64_t sub_pack(int lcc, int y, int cl, const int *n6)
+{
+    uint64_t w = 0;
+    for (int k = 0; k < 6; ++k) w = ryr_set(w, k, n6[k]);
+    w = sub_set_lcc(w, lcc);
+    if (y) w |= 1ULL << 52;
+    if (cl) w |= 1ULL << 53;
+    return w;
+}
 
 // New LCC state after transition `tr` from state s (the map of sample_LCC,
 // src/includes/SSA.tpp:103-197): mode-normal states 1..6, mode-Ca 7..12.
@@ -94,14 +107,13 @@ GW_HD void ryr_rates(const SsaParams &P, uint64_t w, double ca, F &&f)
 
 // Subunit total in the reference accumulation order (SSA.tpp:48-56):
 // 0 + LCC sum, + inactivation, + RyR[0..11], + ClCh.
-GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, const LccTable &T, uint32_t chan, uint64_t w,
-                           int j, double ca)
+GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, const LccTable &T, uint64_t w, double ca)
 {
-    const int L = chan_lcc(chan, j);
+    const int L = sub_lcc(w);
     double s = T.P[L] + T.Q[L] * ca;
-    s += chan_y(chan, j) ? K.inact1 : K.inact0;
+    s += sub_y(w) ? K.inact1 : K.inact0;
     ryr_rates(P, w, ca, [&](int, double r) { s += r; });
-    s += chan_cl(chan, j) ? P.kbClCh : P.kfClCh * ca;
+    s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
     return s;
 }
 
@@ -110,76 +122,55 @@ GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, const LccTab
 // drawing from a stream keyed by the complement of the CRU's SSA stream key.
 // Per CRU the draws are LCC[0..3], y[0..3], RyR[0..3][0..4], ClCh[0..3].
 template <int PRNG>
-GW_HD void sample_rest_cru(uint64_t seed, int64_t i, double ca[4], double &jsr, uint32_t &chan, uint64_t ryr[4])
+GW_HD void sample_rest_cru(uint64_t seed, int64_t i, double ca[4], double &jsr, uint64_t sub[4])
 {
     uint64_t s[4];
     rng_seed(PRNG, s, ~stream_key(seed, i));
     auto u = [&]() { return u01(rng_next<PRNG>(s[0], s[1], s[2], s[3])); };
-    chan = 0;
+    int lcc[4], y[4], cl[4], n[4][6];
     for (int j = 0; j < 4; ++j) {
         const double v = u();  // sample_weights over {0.958, 0.038, 0.004} -> LCC 1, 2, 7
         int idx = 0;
         double cum = 0.958;
         if (cum < v) { idx = 1; cum += 0.038; if (cum < v) idx = 2; }
-        chan = chan_set_lcc(chan, j, idx == 0 ? 1 : (idx == 1 ? 2 : 7));
+        lcc[j] = idx == 0 ? 1 : (idx == 1 ? 2 : 7);
     }
     for (int j = 0; j < 4; ++j) {
         const double v = u();  // {0.9425, 0.0575} -> y = 1, 0
-        if (!(0.9425 < v)) chan |= 1u << (16 + j);
+        y[j] = (0.9425 < v) ? 0 : 1;
     }
     for (int j = 0; j < 4; ++j) {
-        uint64_t w = 0;
+        for (int k = 0; k < 6; ++k) n[j][k] = 0;
         for (int k = 0; k < 5; ++k) {
             const double v = u();  // {0.609, 0.1955, 0.1955} -> RyR states 1, 5, 6
             int idx = 0;
             double cum = 0.609;
             if (cum < v) { idx = 1; cum += 0.5 * 0.391; if (cum < v) idx = 2; }
-            w = ryr_add(w, idx == 0 ? 0 : (idx == 1 ? 4 : 5), 1);
+            n[j][idx == 0 ? 0 : (idx == 1 ? 4 : 5)] += 1;
         }
-        ryr[j] = w;
     }
     for (int j = 0; j < 4; ++j) {
         const double v = u();  // {0.998, 0.002} -> ClCh 0, 1
-        if (0.998 < v) chan |= 1u << (20 + j);
+        cl[j] = (0.998 < v) ? 1 : 0;
     }
-    for (int j = 0; j < 4; ++j) ca[j] = 1.45370e-4;
+    for (int j = 0; j < 4; ++j) {
+        ca[j] = 1.45370e-4;
+        sub[j] = sub_pack(lcc[j], y[j], cl[j], n[j]);
+    }
     jsr = 0.908408;
 }
 
-GW_HD void sample_rest_cru_any(int prng, uint64_t seed, int64_t i, double ca[4], double &jsr, uint32_t &chan,
-                               uint64_t ryr[4])
+GW_HD void sample_rest_cru_any(int prng, uint64_t seed, int64_t i, double ca[4], double &jsr, uint64_t sub[4])
 {
     switch (prng) {
-    case 0: sample_rest_cru<0>(seed, i, ca, jsr, chan, ryr); break;
-    case 1: sample_rest_cru<1>(seed, i, ca, jsr, chan, ryr); break;
-    case 2: sample_rest_cru<2>(seed, i, ca, jsr, chan, ryr); break;
-    case 3: sample_rest_cru<3>(seed, i, ca, jsr, chan, ryr); break;
-    case 4: sample_rest_cru<4>(seed, i, ca, jsr, chan, ryr); break;
-    default: sample_rest_cru<5>(seed, i, ca, jsr, chan, ryr); break;
+    case 0: sample_rest_cru<0>(seed, i, ca, jsr, sub); break;
+    case 1: sample_rest_cru<1>(seed, i, ca, jsr, sub); break;
+    case 2: sample_rest_cru<2>(seed, i, ca, jsr, sub); break;
+    case 3: sample_rest_cru<3>(seed, i, ca, jsr, sub); break;
+    case 4: sample_rest_cru<4>(seed, i, ca, jsr, sub); break;
+    default: sample_rest_cru<5>(seed, i, ca, jsr, sub); break;
     }
 }
-
-// Register-resident CRU.
-struct Cru64 {
-    double ca[4];
-    double jsr;
-    double jl[4];
-    uint32_t chan;
-    uint64_t ryr[4];
-    uint64_t r0, r1, r2, r3;   // engine state
-};
-
-// Per-CRU outputs of one step for the global reductions.
-struct CruOut {
-    double Jxfer;   // ((Jxfer0 + Jxfer1) + Jxfer2) + Jxfer3 of the last iteration
-    double Jtr;     // Jtr of the last iteration
-    double JLCC;    // ((JLCC0 + JLCC1) + JLCC2) + JLCC3 after the step
-    double Jrel;    // sum of Jrel after the step (observable)
-    int open;       // open RyRs (states 3 + 4) after the step
-    int clch;       // open ClCh after the step
-    uint32_t iters; // SSA loop iterations
-    int err;        // nonzero if the iteration cap was hit
-};
 
 struct Digest {
     uint64_t count, hash;
@@ -193,93 +184,131 @@ GW_HD void digest_event(Digest &d, uint32_t step_lo, int j, int kind, int detail
     d.hash = (d.hash ^ code) * 0x100000001b3ULL;
 }
 
-template <int PRNG>
-GW_HD double draw(Cru64 &c) { return u01(rng_next<PRNG>(c.r0, c.r1, c.r2, c.r3)); }
-
-// 4-way register select helpers (no dynamically indexed private arrays).
-GW_HD double sel4(const double *a, int j) { return j == 0 ? a[0] : j == 1 ? a[1] : j == 2 ? a[2] : a[3]; }
-GW_HD uint64_t sel4(const uint64_t *a, int j) { return j == 0 ? a[0] : j == 1 ? a[1] : j == 2 ? a[2] : a[3]; }
-
 constexpr uint32_t kMaxItersPerStep = 1u << 22;
 
-// One global step of SSA_single_CRU (SSA.tpp:286-310) on a register-resident CRU.
-template <int PRNG, bool DIGEST>
-GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, const LccTable &T, double dt, Digest &dig,
-                      uint32_t step_lo)
+// ---------------------------------------------------------------------------
+// Quad helpers: DPP quad_perm moves within each group of 4 lanes. Every call
+// sits in quad-uniform control flow, so the source lanes are active.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ uint32_t qmov32(uint32_t v)
 {
-    CruOut out;
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t qmov64(uint64_t u)
+{
+    return (static_cast<uint64_t>(qmov32<CTRL>(static_cast<uint32_t>(u >> 32))) << 32) |
+           qmov32<CTRL>(static_cast<uint32_t>(u));
+}
+template <int CTRL>
+__device__ __forceinline__ double qmovd(double v) { return double_of(qmov64<CTRL>(bits_of(v))); }
+template <int CTRL>
+__device__ __forceinline__ int qmovi(int v) { return static_cast<int>(qmov32<CTRL>(static_cast<uint32_t>(v))); }
+
+template <int K>
+__device__ __forceinline__ double qgetd(double v) { return qmovd<K * 0x55>(v); }   // lane K of the quad
+template <int K>
+__device__ __forceinline__ int qgeti(int v) { return qmovi<K * 0x55>(v); }
+__device__ __forceinline__ double qnext(double v) { return qmovd<0x39>(v); }      // lane (j + 1) & 3
+__device__ __forceinline__ double qprev(double v) { return qmovd<0x93>(v); }      // lane (j + 3) & 3
+__device__ __forceinline__ int qseli(int v, int k)                                // lane k (quad-uniform k)
+{
+    const int a = qgeti<0>(v), b = qgeti<1>(v), c = qgeti<2>(v), d = qgeti<3>(v);
+    return k == 0 ? a : k == 1 ? b : k == 2 ? c : d;
+}
+__device__ __forceinline__ uint64_t qselu(uint64_t v, int k)
+{
+    const uint64_t a = qmov64<0x00>(v), b = qmov64<0x55>(v), c = qmov64<0xAA>(v), d = qmov64<0xFF>(v);
+    return k == 0 ? a : k == 1 ? b : k == 2 ? c : d;
+}
+__device__ __forceinline__ double qsum4(double v) { return qgetd<0>(v) + qgetd<1>(v) + qgetd<2>(v) + qgetd<3>(v); }
+__device__ __forceinline__ int qsum4i(int v) { return qgeti<0>(v) + qgeti<1>(v) + qgeti<2>(v) + qgeti<3>(v); }
+
+// Replicated CRU-level state of a quad.
+struct CruQ {
+    double jsr;
+    uint64_t r0, r1, r2, r3;
+};
+
+// Per-CRU outputs of one step for the reductions (identical in the 4 lanes).
+struct QuadOut {
+    double Jxfer;    // ((Jxfer0 + Jxfer1) + Jxfer2) + Jxfer3 of the last iteration
+    double Jtr;      // Jtr of the last iteration
+    double JLCC;     // ((JLCC0 + JLCC1) + JLCC2) + JLCC3 after the step
+    double Jrel;     // sum of Jrel after the step
+    int open;        // open RyRs (states 3 + 4) after the step
+    int clch;        // open ClCh after the step
+    uint32_t iters;  // SSA loop iterations
+    int err;         // nonzero if the iteration cap was hit
+};
+
+template <int PRNG>
+__device__ __forceinline__ double qdraw(CruQ &q) { return u01(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3)); }
+
+// One global step of SSA_single_CRU (SSA.tpp:286-310); lane j holds subunit j.
+template <int PRNG, bool DIGEST>
+__device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, const SsaParams &P,
+                                 const StepConsts &K, const LccTable &T, double dt, Digest &dig, uint32_t step_lo,
+                                 int j)
+{
+    QuadOut out;
     out.iters = 0;
     out.err = 0;
-    // copy_from_CRUState (SSA.tpp:10-33): open counts and Jrel at the step start
-    double Jrel[4];
-    int open[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        open[j] = ryr_get(c.ryr[j], 2) + ryr_get(c.ryr[j], 3);
-        Jrel[j] = P.rRyR * static_cast<double>(open[j]) * (c.jsr - c.ca[j]);
-    }
-    double t = 0.0;
-    double old[4];
-    double Jtr = 0.0;
+    // copy_from_CRUState (SSA.tpp:10-33)
+    int open = ryr_get(w, 2) + ryr_get(w, 3);
+    double Jrel = P.rRyR * static_cast<double>(open) * (q.jsr - ca);
+    double t = 0.0, Jtr = 0.0, Jxfer = 0.0;
     while (true) {
         out.iters++;
         // update_fluxes + update_rates (SSA.tpp:37-56)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) old[j] = c.ca[j];
-        Jtr = P.rtr * (K.CaNSR - c.jsr);
-        double s[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, c.chan, c.ryr[j], j, old[j]);
-        const double total = s[0] + s[1] + s[2] + s[3];
-        const double jump = -gw::log(draw<PRNG>(c)) / total;
+        const double old = ca;
+        const double o_next = qnext(old);
+        const double o_prev = qprev(old);
+        Jtr = P.rtr * (K.CaNSR - q.jsr);
+        const double s_me = subunit_total(P, K, T, w, old);
+        const double s0 = qgetd<0>(s_me), s1 = qgetd<1>(s_me), s2 = qgetd<2>(s_me), s3 = qgetd<3>(s_me);
+        const double total = s0 + s1 + s2 + s3;
+        const double jump = -gw::log(qdraw<PRNG>(q)) / total;
         const bool last = !(t + jump < dt);
         const double h = last ? dt - t : jump;
-        const uint64_t ryr_pre[4] = {c.ryr[0], c.ryr[1], c.ryr[2], c.ryr[3]};
-        // update_CaSS (SSA.tpp:87-99)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const double Jiss = P.riss * (old[(j + 1) & 3] + old[(j + 3) & 3] - 2 * old[j]);
-            const double Jxfer = P.rxfer * (old[j] - K.Cai);
-            const double betaSS = 1.0 / (1 + (P.BSR_const / sq(P.KBSR + old[j])) + (P.BSL_const / sq(P.KBSL + old[j])));
-            const double dCaSS = h * (c.jl[j] + Jrel[j] - Jxfer + Jiss) * betaSS;
-            const double tmp = old[j] + dCaSS;
-            if (old[j] > 1.15e-4 && tmp <= 1.15e-4) {
-                // sample_RyR56 (SSA.tpp:59-71)
-                uint64_t w = c.ryr[j];
-                const int n56 = ryr_get(w, 4) + ryr_get(w, 5);
-                if (n56 > 0) {
-                    const double pr = P.k65 / (P.k65 + P.k56 * sq(old[j]));
+        const uint64_t w_pre = w;
+        // update_CaSS (SSA.tpp:87-99) for this lane's subunit
+        const double Jiss = P.riss * (o_next + o_prev - 2 * old);
+        Jxfer = P.rxfer * (old - K.Cai);
+        const double betaSS = 1.0 / (1 + (P.BSR_const / sq(P.KBSR + old)) + (P.BSL_const / sq(P.KBSL + old)));
+        const double dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
+        const double tmp = old + dCaSS;
+        const int split = (old > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((old > 0.03685 && tmp <= 0.03685) ? 2 : 0);
+        const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
+        if (any_split) {
+            // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
+            // lane draws the same Bernoulli uniforms and the owner applies them
+            const uint64_t ob = bits_of(old);
+            for (int jj = 0; jj < 4; ++jj) {
+                const int sj = qseli(split, jj);
+                if (sj) {
+                    const int a_ = (sj == 1) ? 4 : 2;
+                    const uint64_t wq = qselu(w, jj);
+                    const double oq = double_of(qselu(ob, jj));
+                    const double pr = (sj == 1) ? P.k65 / (P.k65 + P.k56 * sq(oq)) : P.k43 / (P.k43 + P.k34 * sq(oq));
+                    const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
                     int X = 0;
-                    for (int i = 0; i < n56; ++i)
-                        if (draw<PRNG>(c) < pr) X++;
-                    w = ryr_set(ryr_set(w, 4, X), 5, n56 - X);
-                } else {
-                    w = ryr_set(ryr_set(w, 4, 0), 5, 0);
+                    for (int i = 0; i < n; ++i)
+                        if (qdraw<PRNG>(q) < pr) X++;
+                    const int na = (n > 0) ? X : 0;
+                    const int nb = (n > 0) ? n - X : 0;
+                    if (j == jj) w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
+                    if (DIGEST) digest_event(dig, step_lo, jj, sj == 1 ? 4 : 5, na);
                 }
-                c.ryr[j] = w;
-                if (DIGEST) digest_event(dig, step_lo, j, 4, ryr_get(w, 4));
-            } else if (old[j] > 0.03685 && tmp <= 0.03685) {
-                // sample_RyR34 (SSA.tpp:73-84)
-                uint64_t w = c.ryr[j];
-                const int n34 = ryr_get(w, 2) + ryr_get(w, 3);
-                if (n34 > 0) {
-                    const double pr = P.k43 / (P.k43 + P.k34 * sq(old[j]));
-                    int X = 0;
-                    for (int i = 0; i < n34; ++i)
-                        if (draw<PRNG>(c) < pr) X++;
-                    w = ryr_set(ryr_set(w, 2, X), 3, n34 - X);
-                } else {
-                    w = ryr_set(ryr_set(w, 2, 0), 3, 0);
-                }
-                c.ryr[j] = w;
-                if (DIGEST) digest_event(dig, step_lo, j, 5, ryr_get(w, 2));
             }
-            c.ca[j] = tmp;
         }
-        // update_CaJSR (SSA.hpp:67-71)
+        ca = tmp;
+        // update_CaJSR (SSA.hpp:67-71), replicated
         {
-            const double betaJSR = 1.0 / (1 + P.CSQN_const / sq(P.KCSQN + c.jsr));
-            c.jsr += (h * betaJSR * (Jtr - P.VSS_VJSR * (Jrel[0] + Jrel[1] + Jrel[2] + Jrel[3])));
+            const double Jsum = qsum4(Jrel);
+            const double betaJSR = 1.0 / (1 + P.CSQN_const / sq(P.KCSQN + q.jsr));
+            q.jsr += (h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
         }
         if (last) break;
         if (out.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
@@ -288,103 +317,87 @@ GW_HD CruOut ssa_step(Cru64 &c, const SsaParams &P, const StepConsts &K, const L
         }
         t += jump;
         // sample_weights over the subunits (common.hpp:49-59)
-        int j;
+        int js;
         {
-            const double u = draw<PRNG>(c) * total;
-            double cum = s[0];
-            j = 0;
-            if (cum < u) { j = 1; cum += s[1];
-                if (cum < u) { j = 2; cum += s[2];
-                    if (cum < u) { j = 3; } } }
+            const double u = qdraw<PRNG>(q) * total;
+            double cum = s0;
+            js = 0;
+            if (cum < u) { js = 1; cum += s1;
+                if (cum < u) { js = 2; cum += s2;
+                    if (cum < u) { js = 3; } } }
         }
-        // sample_new_state (SSA.tpp:262-283) with the propensities as stored
-        // before this iteration's CaSS update
-        const double caj = sel4(old, j);
-        const double sj = sel4(s, j);
-        const int L = chan_lcc(c.chan, j);
-        const int y = chan_y(c.chan, j);
-        const int cl = chan_cl(c.chan, j);
-        const double sum_LCC = T.P[L] + T.Q[L] * caj;
+        // sample_new_state (SSA.tpp:262-283): every lane classifies its own
+        // subunit; the chosen lane's class is broadcast to the quad
+        const double u3 = qdraw<PRNG>(q);
+        const int L = sub_lcc(w);
+        const int y = sub_y(w);
+        const int cl = sub_cl(w);
+        const double sum_LCC = T.P[L] + T.Q[L] * old;
         const double inact = y ? K.inact1 : K.inact0;
-        const double clr = cl ? P.kbClCh : P.kfClCh * caj;
-        const double sum_RyR = sj - (sum_LCC + inact + clr);
-        const double u = draw<PRNG>(c) * sj;
-        const double ca_new = sel4(c.ca, j);
-        double jl_new = sel4(c.jl, j);
-        if (u < sum_LCC) {
-            // sample_LCC (SSA.tpp:103-197)
-            // the scan over (r0, r1, r2) never compares after adding r2, so r2 is not needed
-            const double r0 = T.R0[L];
-            const double r1 = (L == 1) ? T.Q[L] * caj : T.R1[L];
-            const double uu = draw<PRNG>(c) * sum_LCC;
-            double cum = r0;
-            int tr = 0;
-            if (cum < uu) { tr = 1; cum += r1;
-                if (cum < uu) { tr = 2; } }
-            const int Ln = lcc_next(L, tr);
-            if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
-                jl_new = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca_new) : 0.0;
-            else if (L == 6 || L == 12)
-                jl_new = 0.0;
-            c.chan = chan_set_lcc(c.chan, j, Ln);
-            if (DIGEST) digest_event(dig, step_lo, j, 0, Ln);
-        } else if (u < (sum_LCC + inact)) {
-            const int yn = 1 - y;
-            c.chan = chan_flip_y(c.chan, j);
-            if (yn == 0) jl_new = 0.0;
-            else if (L == 6 || L == 12) jl_new = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca_new);
-            if (DIGEST) digest_event(dig, step_lo, j, 1, yn);
-        } else if (u < (sum_LCC + inact + sum_RyR)) {
-            // sample_RyR (SSA.tpp:200-259)
-            const double uu = draw<PRNG>(c) * sum_RyR;
-            int tr = 0;
-            double cum = 0.0;
-            bool done = false;
-            ryr_rates(P, sel4(ryr_pre, j), caj, [&](int k, double r) {
-                if (done) return;
-                if (k == 0) { cum = r; }
-                else { if (cum < uu) { tr = k; cum += r; } else done = true; }
-            });
-            // transition (from, to) in storage order, applied to the current counts
-            const int from = (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
-                           : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
-            const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
-                         : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
-            uint64_t w = sel4(c.ryr, j);
-            w = ryr_add(ryr_add(w, from, -1), to, +1);
-            int dopen = 0;
-            if (tr == 1 || tr == 10) dopen = 1;
-            else if (tr == 4 || tr == 7) dopen = -1;
-            const int oj = (j == 0 ? open[0] : j == 1 ? open[1] : j == 2 ? open[2] : open[3]) + dopen;
-            const double jr = P.rRyR * static_cast<double>(oj) * (c.jsr - ca_new);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q == j) { c.ryr[q] = w; open[q] = oj; Jrel[q] = jr; }
-            if (DIGEST) digest_event(dig, step_lo, j, 2, tr);
-        } else {
-            c.chan = chan_flip_cl(c.chan, j);
-            if (DIGEST) digest_event(dig, step_lo, j, 3, 1 - cl);
+        const double clr = cl ? P.kbClCh : P.kfClCh * old;
+        const double sum_RyR = s_me - (sum_LCC + inact + clr);
+        const double u = u3 * s_me;
+        const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
+        const int cls_q = qseli(cls, js);
+        const double u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG>(q) : 0.0;
+        int detail = 0;
+        if (j == js) {
+            if (cls == 0) {
+                // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
+                const double r0 = T.R0[L];
+                const double r1 = (L == 1) ? T.Q[L] * old : T.R1[L];
+                const double uu = u4 * sum_LCC;
+                double cum = r0;
+                int tr = 0;
+                if (cum < uu) { tr = 1; cum += r1;
+                    if (cum < uu) { tr = 2; } }
+                const int Ln = lcc_next(L, tr);
+                if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
+                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca) : 0.0;
+                else if (L == 6 || L == 12)
+                    jl = 0.0;
+                w = sub_set_lcc(w, Ln);
+                detail = Ln;
+            } else if (cls == 1) {
+                const int yn = 1 - y;
+                w = sub_flip_y(w);
+                if (yn == 0) jl = 0.0;
+                else if (L == 6 || L == 12) jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca);
+                detail = yn;
+            } else if (cls == 2) {
+                // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
+                const double uu = u4 * sum_RyR;
+                int tr = 0;
+                double cum = 0.0;
+                bool done = false;
+                ryr_rates(P, w_pre, old, [&](int k, double r) {
+                    if (done) return;
+                    if (k == 0) { cum = r; }
+                    else { if (cum < uu) { tr = k; cum += r; } else done = true; }
+                });
+                const int from = (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
+                               : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
+                const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
+                             : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
+                w = ryr_add(ryr_add(w, from, -1), to, +1);
+                if (tr == 1 || tr == 10) open += 1;
+                else if (tr == 4 || tr == 7) open -= 1;
+                Jrel = P.rRyR * static_cast<double>(open) * (q.jsr - ca);
+                detail = tr;
+            } else {
+                w = sub_flip_cl(w);
+                detail = 1 - cl;
+            }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q == j) c.jl[q] = jl_new;
+        if (DIGEST) digest_event(dig, step_lo, js, cls_q, qseli(detail, js));
     }
     // outputs of update_CRUstate_from_temp (GW.tpp:47-62) for the reductions
-    double jx[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) jx[j] = P.rxfer * (old[j] - K.Cai);
-    out.Jxfer = jx[0] + jx[1] + jx[2] + jx[3];
+    out.Jxfer = qsum4(Jxfer);
     out.Jtr = Jtr;
-    out.JLCC = c.jl[0] + c.jl[1] + c.jl[2] + c.jl[3];
-    out.Jrel = Jrel[0] + Jrel[1] + Jrel[2] + Jrel[3];
-    int op = 0, cl = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        op += ryr_get(c.ryr[j], 2) + ryr_get(c.ryr[j], 3);
-        cl += chan_cl(c.chan, j);
-    }
-    out.open = op;
-    out.clch = cl;
+    out.JLCC = qsum4(jl);
+    out.Jrel = qsum4(Jrel);
+    out.open = qsum4i(ryr_get(w, 2) + ryr_get(w, 3));
+    out.clch = qsum4i(sub_cl(w));
     return out;
 }
 
