@@ -10,9 +10,14 @@ with HIP events on the library's stream.
 
   value        CRU*ms simulated per second over the whole batch
   cell_beats   cells * (simulated ms / 1000 ms per beat) per second
-  roofline     algorithmic FP64 flops (F_ITER per SSA iteration x iterations
-               counted in-kernel) per launch / launch time, against the FP64
-               vector peak; state bytes per launch against HBM
+  roofline     for the SSA kernel k_ssa: algorithmic FP64 flops per launch
+               (F_ITER per SSA iteration x iterations counted in-kernel) and
+               algorithmic state bytes per launch (STATE_BYTES per CRU-step),
+               each divided by the kernel's average duration from HIP events
+               recorded by every SSA dispatch of the timed region; against the
+               FP64 vector peak and HBM. `traffic` is the measured HBM bytes per
+               launch from profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and
+               WRITE_SIZE, calibrated on a copy of known size) when present.
   cpu_baseline the CPU restatement built with the reference's release flags,
                OpenMP over CRUs on all host cores, on a bounded sample
 """
@@ -31,7 +36,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "gw-model_amd")
 
-F_ITER = 330            # FP64 flops of one SSA loop iteration (count in DESIGN.md)
+# FP64 operations of one SSA loop iteration, counted on the oracle's
+# SSA_single_CRU (add, sub, mul, div, int->fp each 1, log 1) for the iteration
+# that closes a step at rest: fluxes 26, rates 4 x 56, total + waiting time 7,
+# dt - t 1, CaSS 4 x 15, CaJSR 13.
+F_ITER = 331
 STATE_BYTES = 272       # bytes read + written per CRU-step by the FP64 SSA kernel: 136 in, 136 out
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -55,13 +64,29 @@ def gpu_bench(capi, cells, crus, steps, warmup, seed0=42):
     if warmup > 0:
         b.step(warmup)
     i0 = b.iterations()
-    ms, launches = b.time_steps(steps)
+    ms, ssa_ms, launches = b.time_steps(steps, per_kernel=True)
     iters = b.iterations() - i0
     g = b.get_globals()
     obs = b.obs()
     b.close()
     ctx.close()
-    return ms, launches, iters, g, obs
+    return ms, ssa_ms, launches, iters, g, obs
+
+
+def measured_traffic(total_crus):
+    """HBM bytes per k_ssa launch from the committed PMC summary, scaled to this batch."""
+    path = os.path.join(ROOT, "profiles", "k_ssa_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path, encoding="utf-8"):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_cru_step"]) * total_crus
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(crus, steps, warmup_steps=0, seed=42):
@@ -104,13 +129,13 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
-    ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup)
+    ms, ssa_ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup)
     total_crus = args.cells * args.crus
     sim_ms = args.steps * 1e-3
     sec = ms / 1e3
     value = total_crus * sim_ms / sec
     cell_beats = args.cells * (sim_ms / 1000.0) / sec
-    per_launch_s = sec / launches
+    per_launch_s = ssa_ms / 1e3 / launches
     flops_per_launch = F_ITER * iters / launches
     achieved_tflops = flops_per_launch / per_launch_s / 1e12
     bytes_per_launch = STATE_BYTES * total_crus
@@ -118,9 +143,9 @@ def main(argv=None):
     frac_f = achieved_tflops / PEAK_FP64_TFLOPS
     frac_b = achieved_gbs / PEAK_HBM_GBS
     line = {
-        "metric": "CRU*ms simulated/s and cell-beats/s, batched FP32-local vs plain FP64, 1 GPU",
+        "metric": "CRU\u00b7ms simulated/s and cell-beats/s, batched FP32-local vs plain FP64, 1 GPU",
         "value": value,
-        "unit": "CRU*ms/s",
+        "unit": "CRU\u00b7ms/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -142,9 +167,10 @@ def main(argv=None):
             "peak": PEAK_FP64_TFLOPS if frac_f >= frac_b else PEAK_HBM_GBS,
             "unit": "TFLOP/s" if frac_f >= frac_b else "GB/s",
             "frac": max(frac_f, frac_b),
-            "traffic": None,
-            "kernel": "k_ssa_step<0,false>",
+            "traffic": measured_traffic(total_crus),
+            "kernel": "k_ssa<0,false>",
             "avg_launch_us": per_launch_s * 1e6,
+            "ssa_share_of_step": (ssa_ms / ms) if ms > 0 else None,
             "fp64": {"achieved_tflops": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "frac": frac_f,
                      "flops_per_launch": flops_per_launch, "f_iter": F_ITER},
             "hbm": {"achieved_gbs": achieved_gbs, "peak": PEAK_HBM_GBS, "frac": frac_b,
@@ -154,7 +180,7 @@ def main(argv=None):
     }
     if not args.no_cpu and rank == 0:
         v, threads, secs = cpu_baseline(args.cpu_crus, args.cpu_steps)
-        line["cpu_baseline"] = {"value": v, "unit": "CRU*ms/s", "cores": threads, "kind": "port",
+        line["cpu_baseline"] = {"value": v, "unit": "CRU\u00b7ms/s", "cores": threads, "kind": "port",
                                 "sample": f"1 cell x {args.cpu_crus} CRUs x {args.cpu_steps} steps "
                                           f"(t = 0..{args.cpu_steps * 1e-3:g} ms of beat 1), {secs:.1f} s",
                                 "ratio_gpu_over_cpu": value / v}

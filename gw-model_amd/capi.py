@@ -137,7 +137,7 @@ def lib() -> C.CDLL:
         "gw_batch_set_mode": (C.c_int, [P, i32, i32]),
         "gw_batch_get_iterations": (C.c_int, [P, C.POINTER(u64)]),
         "gw_batch_set_launch": (C.c_int, [P, i32, i32] + E),
-        "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(i64)] + E),
+        "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
     }
     for name, (res, args) in sig.items():
@@ -307,10 +307,12 @@ class Batch:
     def set_launch(self, crus_per_thread: int = 0, block: int = 0):
         _call(self.L.gw_batch_set_launch, self.h, crus_per_thread, block)
 
-    def time_steps(self, nsteps: int):
-        ms, launches = f64(), i64()
-        _call(self.L.gw_batch_time_steps, self.h, int(nsteps), C.byref(ms), C.byref(launches))
-        return ms.value, launches.value
+    def time_steps(self, nsteps: int, per_kernel: bool = False):
+        """(total ms, SSA-kernel ms or None, SSA launches) of nsteps Euler steps."""
+        ms, ssa, launches = f64(), f64(), i64()
+        _call(self.L.gw_batch_time_steps, self.h, int(nsteps), C.byref(ms), C.byref(ssa) if per_kernel else None,
+              C.byref(launches))
+        return ms.value, (ssa.value if per_kernel else None), launches.value
 
 
 def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, istim, record_every: int,

@@ -8,13 +8,19 @@
 // of lanes (lane j owns subunit j), so one 256-thread block holds 64 CRUs; each
 // cell's slots are padded to whole blocks, so a block belongs to one cell.
 //
-// One Euler step (GW_model::euler_step, src/includes/GW.tpp:236-272) is two
-// launches: k_ssa advances every CRU through SSA(dt) with the cell's frozen
-// constants and reduces the flux sums exactly (128-bit fixed point) into one
-// partial per block; k_cell_close (one wave per cell) sums the partials,
-// applies globals += dGlobals and prepares the next step: derivatives at the
-// new state and the SSA constants.
+// One Euler step (GW_model::euler_step, src/includes/GW.tpp:236-272) is one
+// launch of k_step. Its SSA blocks advance every CRU through SSA(dt) with the
+// cell's frozen constants and reduce the flux sums exactly (128-bit fixed
+// point) into one partial per block. Its first blocks, one per cell, close
+// the previous step at the same time: they sum that step's partials, apply
+// globals += dGlobals, form the derivatives of the new step and the SSA
+// constants of the step after it. This works because SSA(n+1) needs only the
+// globals of step n+1 = globals(n) + dGlobals(n), and dGlobals(n) was formed
+// from the sums of step n-1; the sums of step n enter dGlobals(n+1) only.
+// Step constants and block partials are double-buffered by step parity.
 #include "gw_kernels.h"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -91,11 +97,11 @@ __device__ void obs_update(gw_cell_obs_t &o, double V_old, double V, double t_ol
     o.v_prev = V;
 }
 
-// Derivatives and SSA constants of the step d.step at state d.g (the part of
-// euler_step before SSA(dt), GW.tpp:238-247 and 216-221).
-// Works on register copies of the cell's globals and constants; the results
-// are written back to the cell's dynamic state once.
-__device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, double dt, Currents &cur)
+// Derivatives of the step d.step at state d.g (the part of euler_step before
+// SSA(dt), GW.tpp:238-247) and, when kslot >= 0, that step's SSA constants
+// (GW.tpp:216-221) into d.k[kslot]. Works on register copies of the cell's
+// globals and constants; the results are written back to the cell's state once.
+__device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, double dt, Currents &cur, int kslot)
 {
     const gw_params_t p = C.p;
     const CellConsts c = C.c;
@@ -107,12 +113,28 @@ __device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, do
     const double expm = gw::exp(-VF_RT);
     const int rc = global_derivatives(p, c, g, VF_RT, expm, dt, Istim, S, dg, cur);
     if (rc != GW_OK && d.status == 0) d.status = rc;
-    StepConsts k;
-    step_consts(k, p, c, g.V, expm, g.Cai, g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
     d.Istim = Istim;
     d.g = g;
     d.dg = dg;
-    d.k = k;
+    if (kslot >= 0) step_consts(d.k[kslot], p, c, g.V, expm, g.Cai, g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+}
+
+// SSA constants of the step after d.step, into d.k[(d.step + 1) & 1]: the
+// potential and concentrations that step will start from (globals + dGlobals
+// unless frozen; the clamp potential at t + dt under a clamp protocol), formed
+// with the same operations prepare_step will apply at that step.
+__device__ void lookahead_consts(const DevCell &C, CellDyn &d, double dt, bool frozen)
+{
+    double V = d.g.V, Cai = d.g.Cai, CaNSR = d.g.CaNSR;
+    if (!frozen) {
+        if (!d.clamp) V += d.dg.V;
+        Cai += d.dg.Cai;
+        CaNSR += d.dg.CaNSR;
+    }
+    if (d.clamp) V = protocol_vclamp(C, d.t + dt);
+    const double VF_RT = V * C.c.F_RT;
+    const double expm = gw::exp(-VF_RT);
+    step_consts(d.k[(d.step + 1) & 1], C.p, C.c, V, expm, Cai, CaNSR, static_cast<uint32_t>((d.step + 1) & 0xffff));
 }
 
 __device__ void sum_block_partials(const Partial *parts, int64_t first, int n, Partial &tot)
@@ -253,16 +275,17 @@ __global__ void k_init_cells(KArgs a)
         D.step = 0;
         D.status = t.err ? GW_E_RANGE : 0;
         Currents cur;
-        prepare_step(C, D, S, a.dt, cur);
+        prepare_step(C, D, S, a.dt, cur, 0);
+        lookahead_consts(C, D, a.dt, a.frozen != 0);
         D.obs.v_prev = D.g.V;
     }
 }
 
-// Copy this step's LCC rate table into LDS (it is read with a per-lane index;
+// Copy a step's LCC rate table into LDS (it is read with a per-lane index;
 // the cell's other constants are wave-uniform and are read as scalars).
-__device__ __forceinline__ void load_lcc_table(const CellDyn &D, LccTable &T)
+__device__ __forceinline__ void load_lcc_table(const StepConsts &K, LccTable &T)
 {
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&D.k.lcc);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&K.lcc);
     uint64_t *dst = reinterpret_cast<uint64_t *>(&T);
     for (int w = threadIdx.x; w < static_cast<int>(sizeof(LccTable) / 8); w += blockDim.x) dst[w] = src[w];
 }
@@ -280,23 +303,22 @@ __device__ __forceinline__ void partial_from(Partial &m, const QuadOut &o, uint6
 }
 
 // SSA(dt) of `nsteps` steps (GW_model::SSA, src/includes/GW.tpp:214-234) for
-// the CRUs of this block with the cell's current constants; nsteps > 1 only in
-// frozen-globals replay. The block's flux partial of the last step goes to
-// partials[block]; only the quad leaders contribute.
+// the 64 CRUs of SSA block b, with the constants of step step0 (nsteps > 1
+// only in frozen-globals replay, where they do not change). The block's flux
+// partial of the last step goes to partials[step0 & 1][b]; only the quad
+// leaders contribute.
 template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock, 4) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
+__device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTable &T, Partial *red)
 {
-    __shared__ LccTable T;
-    __shared__ Partial red[kBlock];
-    const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int j = tid & 3;
     const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const int ks = static_cast<int>(step0 & 1);
     const DevCell &C = a.cells[cell];
     const CellDyn &D = a.dyn[cell];
     const SsaParams &sp = C.sp;
-    const StepConsts &kk = D.k;
-    load_lcc_table(D, T);
+    const StepConsts &kk = D.k[ks];
+    load_lcc_table(kk, T);
     __syncthreads();
     const int64_t i = (b - C.blk0) * kCruPerBlock + (tid >> 2);
     const bool active = (i < C.n_cru) && (D.status == 0);
@@ -342,22 +364,24 @@ __global__ __launch_bounds__(kBlock, 4) void k_ssa(KArgs a, int64_t step0, int64
         }
     }
     block_reduce<kBlock>(red, mine);
-    if (tid == 0) a.partials[b] = red[0];
+    if (tid == 0) a.partials[ks * a.nblocks + b] = red[0];
 }
 
-// Close the step of every cell (one 64-lane block per cell): the exact sum of
-// the cell's block partials, globals += dGlobals (GW.tpp:252-271), t += dt,
-// then the next step's derivatives and SSA constants, observables and trace.
-__global__ __launch_bounds__(kWave) void k_cell_close(KArgs a)
+// Close step n = D.step of one cell: the exact sum of the cell's block
+// partials of step n, globals += dGlobals (GW.tpp:252-271), t += dt, the
+// derivatives of step n+1, observables and trace, and the SSA constants of
+// step n+2 (those of step n+1 were formed one close earlier and are in use by
+// the SSA blocks running beside this one).
+template <int NT>
+__device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *red)
 {
-    __shared__ Partial red[kWave];
-    const int cell = blockIdx.x;
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
+    const int ks = static_cast<int>(D.step & 1);
     Partial mine;
     partial_zero(mine);
-    for (int q = threadIdx.x; q < C.nblk; q += kWave) partial_add(mine, a.partials[C.blk0 + q]);
-    block_reduce<kWave>(red, mine);
+    for (int q = threadIdx.x; q < C.nblk; q += NT) partial_add(mine, a.partials[ks * a.nblocks + C.blk0 + q]);
+    block_reduce<NT>(red, mine);
     if (threadIdx.x != 0) return;
     const Partial &tot = red[0];
     if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
@@ -376,7 +400,8 @@ __global__ __launch_bounds__(kWave) void k_cell_close(KArgs a)
     D.t += a.dt;
     D.step += 1;
     Currents cur;
-    prepare_step(C, D, S, a.dt, cur);
+    prepare_step(C, D, S, a.dt, cur, -1);
+    lookahead_consts(C, D, a.dt, a.frozen != 0);
     if (!(D.g.V == D.g.V) || !(D.g.Cai == D.g.Cai)) {
         if (D.status == 0) D.status = GW_E_NAN;
     }
@@ -392,23 +417,49 @@ __global__ __launch_bounds__(kWave) void k_cell_close(KArgs a)
     }
 }
 
+// One launch per Euler step: the first `nclose` blocks close step ssa_step-1
+// of every cell while the remaining blocks run SSA(ssa_step). The two halves
+// share no data: the SSA reads the constants of its step (formed by the close
+// one launch earlier) and writes the other partials buffer. ssa_step < 0
+// launches the closes alone (the last close of a sequence).
+template <int PRNG, bool DIGEST>
+__global__ __launch_bounds__(kBlock, 4) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
+{
+    __shared__ LccTable T;
+    __shared__ Partial red[kBlock];
+    const int bid = blockIdx.x;
+    if (bid < nclose) {
+        close_cell<kBlock>(a, bid, red);
+        return;
+    }
+    ssa_block<PRNG, DIGEST>(a, bid - nclose, ssa_step, 1, T, red);
+}
+
+// Frozen-globals replay: nsteps SSA steps in one launch.
+template <int PRNG, bool DIGEST>
+__global__ __launch_bounds__(kBlock, 4) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
+{
+    __shared__ LccTable T;
+    __shared__ Partial red[kBlock];
+    ssa_block<PRNG, DIGEST>(a, blockIdx.x, step0, nsteps, T, red);
+}
+
 // Close a frozen replay of nsteps: sums of the last step, clock advanced as
 // t += dt per step (src/pyGW.hpp:116).
-__global__ void k_frozen_close(KArgs a, int64_t nsteps)
+__global__ void k_frozen_close(KArgs a, int64_t step0, int64_t nsteps)
 {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.ncells) return;
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
     Partial tot;
-    sum_block_partials(a.partials, C.blk0, C.nblk, tot);
+    sum_block_partials(a.partials + (step0 & 1) * a.nblocks, C.blk0, C.nblk, tot);
     if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
     if (tot.err && D.status == 0) D.status = GW_E_RANGE;
     D.sums[0] = fx_to_double(tot.jlcc); D.sums[1] = fx_to_double(tot.jxfer); D.sums[2] = fx_to_double(tot.jtr);
     D.sums[3] = static_cast<double>(tot.clch); D.sums[4] = static_cast<double>(tot.open); D.sums[5] = fx_to_double(tot.jrel);
     for (int64_t s = 0; s < nsteps; ++s) D.t += a.dt;
     D.step += nsteps;
-    D.k.step_lo = static_cast<uint32_t>(D.step & 0xffff);
 }
 
 // Packed device state <-> row-major host layout of one cell (GWCRUState, src/pyGW.cpp:78-87).
@@ -589,7 +640,7 @@ void launch_frozen(gw_batch *B, int64_t step0, int64_t n)
         hipLaunchKernelGGL((k_ssa<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
     else
         hipLaunchKernelGGL((k_ssa<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
-    hipLaunchKernelGGL(k_frozen_close, dim3((B->ncells + 63) / 64), dim3(64), 0, B->ctx->stream, B->a, n);
+    hipLaunchKernelGGL(k_frozen_close, dim3((B->ncells + 63) / 64), dim3(64), 0, B->ctx->stream, B->a, step0, n);
 }
 
 void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
@@ -604,28 +655,82 @@ void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
     }
 }
 
+// One launch of k_step: SSA(ssa_step) (none if ssa_step < 0) beside the
+// closes of the previous step (when nclose > 0). When e0/e1 are given, the
+// dispatch itself records them (its start and end, no extra queue packets).
 template <int PRNG>
-void launch_step(gw_batch *B, int64_t step)
+void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
 {
+    const unsigned grid = static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
     if (B->digest)
-        hipLaunchKernelGGL((k_ssa<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step, int64_t(1));
+        hipExtLaunchKernelGGL((k_step<PRNG, true>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
     else
-        hipLaunchKernelGGL((k_ssa<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step, int64_t(1));
-    hipLaunchKernelGGL(k_cell_close, dim3(B->ncells), dim3(kWave), 0, B->ctx->stream, B->a);
+        hipExtLaunchKernelGGL((k_step<PRNG, false>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
 }
 
+void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
+{
+    switch (B->prng) {
+    case 0: launch_k_step<0>(B, ssa_step, nclose, e0, e1); break;
+    case 1: launch_k_step<1>(B, ssa_step, nclose, e0, e1); break;
+    case 2: launch_k_step<2>(B, ssa_step, nclose, e0, e1); break;
+    case 3: launch_k_step<3>(B, ssa_step, nclose, e0, e1); break;
+    case 4: launch_k_step<4>(B, ssa_step, nclose, e0, e1); break;
+    default: launch_k_step<5>(B, ssa_step, nclose, e0, e1); break;
+    }
+}
+
+// Steps step0 .. step0+n-1: n launches, launch k running SSA(step0+k) beside
+// close(step0+k-1), then one launch with the last closes. Between calls every
+// step is closed, so the cell state read by the host is that of step0+n.
 void launch_steps(gw_batch *B, int64_t step0, int64_t n)
 {
-    for (int64_t s = step0; s < step0 + n; ++s) {
-        switch (B->prng) {
-        case 0: launch_step<0>(B, s); break;
-        case 1: launch_step<1>(B, s); break;
-        case 2: launch_step<2>(B, s); break;
-        case 3: launch_step<3>(B, s); break;
-        case 4: launch_step<4>(B, s); break;
-        default: launch_step<5>(B, s); break;
+    for (int64_t k = 0; k < n; ++k) launch_one(B, step0 + k, k == 0 ? 0 : B->ncells, nullptr, nullptr);
+    if (n > 0) launch_one(B, -1, B->ncells, nullptr, nullptr);
+}
+
+// launch_steps with every step's dispatch timed through a ring of event pairs:
+// before a pair is reused, the host waits for the dispatch that last recorded
+// it (kRing steps back), so the queue never runs dry. The final close-only
+// launch is not timed.
+int launch_steps_timed(gw_batch *B, int64_t step0, int64_t n, double *ssa_ms, char *err, size_t errlen)
+{
+    constexpr int kRing = 512;
+    std::vector<hipEvent_t> ev(2 * kRing, nullptr);
+    int rc = GW_OK;
+    double acc = 0.0;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            set_err(err, errlen, "hipEventCreate failed");
+            rc = GW_E_HIP;
+            break;
         }
+    auto harvest = [&](int r) {
+        float f = 0.f;
+        if (hipEventSynchronize(ev[2 * r + 1]) != hipSuccess || hipEventElapsedTime(&f, ev[2 * r], ev[2 * r + 1]) != hipSuccess)
+            return false;
+        acc += f;
+        return true;
+    };
+    for (int64_t k = 0; rc == GW_OK && k < n; ++k) {
+        const int r = static_cast<int>(k % kRing);
+        if (k >= kRing && !harvest(r)) {
+            set_err(err, errlen, "event timing failed");
+            rc = GW_E_HIP;
+            break;
+        }
+        launch_one(B, step0 + k, k == 0 ? 0 : B->ncells, ev[2 * r], ev[2 * r + 1]);
     }
+    if (rc == GW_OK && n > 0) launch_one(B, -1, B->ncells, nullptr, nullptr);
+    for (int64_t k = std::max<int64_t>(0, n - kRing); rc == GW_OK && k < n; ++k)
+        if (!harvest(static_cast<int>(k % kRing))) {
+            set_err(err, errlen, "event timing failed");
+            rc = GW_E_HIP;
+        }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ssa_ms) *ssa_ms = acc;
+    return rc;
 }
 
 int check_status(gw_batch *B, char *err, size_t errlen)
@@ -870,7 +975,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
         (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
-        (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, B->nblocks, err, errlen)) ||
+        (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, 2 * B->nblocks, err, errlen)) ||
         (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
         (rc = dmalloc(&B->d_block_cell, B->nblocks, err, errlen))) {
@@ -881,6 +986,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     a.dyn = B->d_dyn;
     a.block_cell = B->d_block_cell;
     a.ntot = N;
+    a.nblocks = B->nblocks;
     a.dt = dt;
     a.ncells = ncells;
     a.frozen = 0;
@@ -1213,7 +1319,8 @@ int gw_batch_set_launch(gw_batch *B, int32_t crus_per_thread, int32_t block, cha
     return GW_OK;
 }
 
-int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, int64_t *launches, char *err, size_t errlen)
+int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms, int64_t *launches, char *err,
+                        size_t errlen)
 {
     if (!B || nsteps <= 0 || !ms) {
         set_err(err, errlen, "gw_batch_time_steps: bad arguments");
@@ -1226,7 +1333,12 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, int64_t *launch
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, B->ctx->stream));
-    launch_steps(B, B->steps_done, nsteps);
+    if (ssa_ms) {
+        rc = launch_steps_timed(B, B->steps_done, nsteps, ssa_ms, err, errlen);
+        if (rc) return rc;
+    } else {
+        launch_steps(B, B->steps_done, nsteps);
+    }
     HIP_TRY(hipGetLastError());
     B->steps_done += nsteps;
     HIP_TRY(hipEventRecord(e1, B->ctx->stream));

@@ -24,11 +24,11 @@ struct DevCell {
     int32_t nblk;
 };
 
-// Dynamic per-cell state, advanced by the last block of each step.
+// Dynamic per-cell state, advanced by the cell's close block of each step.
 struct CellDyn {
     gw_globals_t g;      // globals at the start of the current step
     gw_globals_t dg;     // increments formed before the current step's SSA
-    StepConsts k;        // SSA constants of the current step
+    StepConsts k[2];     // SSA constants of steps with even / odd index
     double t;            // accumulated time at the start of the current step
     double Istim;        // stimulus of the current step
     double sums[6];      // JLCC, Jxfer, Jtr, ClCh, open RyR, Jrel of the last SSA
@@ -58,10 +58,11 @@ struct KArgs {
     uint64_t *sub;     // [ntot][4] packed subunit words (RyR counts, LCC, y, ClCh)
     uint64_t *rng;     // [ntot][4] engine words
     uint64_t *dig;     // [ntot][2] (count, hash) when digests are kept
-    Partial *partials; // [nblocks]
+    Partial *partials; // [2][nblocks], by step parity
     double *trace;     // [cap][ncells][4]
     unsigned long long *iters_total;
     int64_t ntot;
+    int64_t nblocks;
     int64_t trace_cap;
     double dt;
     int32_t ncells;

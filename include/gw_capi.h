@@ -285,10 +285,12 @@ int gw_batch_get_iterations(gw_batch *b, uint64_t *iters);
 int gw_batch_set_launch(gw_batch *b, int32_t crus_per_thread, int32_t block,
                         char *err, size_t errlen);
 /*
- * Time nsteps of the SSA/step kernels with HIP events on the batch's stream;
- * returns total milliseconds and the number of kernel launches timed.
+ * Time nsteps Euler steps with HIP events on the batch's stream: *ms is the
+ * wall time of the whole sequence, *launches the number of SSA dispatches.
+ * When ssa_ms is not null, every SSA dispatch also records its own start/end
+ * events and *ssa_ms is the sum of their durations.
  */
-int gw_batch_time_steps(gw_batch *b, int64_t nsteps, double *ms, int64_t *launches,
+int gw_batch_time_steps(gw_batch *b, int64_t nsteps, double *ms, double *ssa_ms, int64_t *launches,
                         char *err, size_t errlen);
 
 /* One-shot convenience: create, run nsteps, copy out globals and observables, destroy. */
