@@ -112,6 +112,25 @@ def cpu_baseline(crus, steps, warmup_steps=0, seed=42):
     return crus * steps * 1e-3 / dt, threads, dt
 
 
+def copy_calibration(nbytes, reps=5):
+    """Device-to-device copies of nbytes (the runtime's copy kernel), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +141,12 @@ def main(argv=None):
     ap.add_argument("--cpu-crus", type=int, default=12500)
     ap.add_argument("--cpu-steps", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(cells are independent and would shard by cell, see DESIGN.md)", file=sys.stderr)
