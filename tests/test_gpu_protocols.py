@@ -1,0 +1,148 @@
+"""GPU parity of the protocol paths and of the drop-in Python surface.
+
+* the voltage-clamp protocol (hold, step, hold), including a step to exactly
+  0 mV, where the LCC flux, Ito2 and IKv14 take their analytic limits;
+* GreensteinWinslow.run (src/pyGW.cpp:341-344) and GWModel.simulate
+  (src/model.py:940-982), checked record by record against the oracle.
+
+All comparisons are exact."""
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import GwGlobals, OracleModel
+
+from .test_gpu_parity import STATE_KEYS, assert_globals_equal, assert_states_equal, say
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    return importlib.import_module("gw-model_amd")
+
+
+@pytest.fixture(scope="module")
+def gw(pkg):
+    import build.GreensteinWinslow as m  # resolved through the package's sys.path entry
+
+    return m
+
+
+@pytest.mark.parametrize("vstep", [10.0, 0.0])
+def test_voltage_clamp_steps_bit_exact(gwcapi, gpu_ctx, vstep):
+    n, steps, t_on, t_off, hold = 200, 4000, 1.0005, 3.0005, -80.0
+    say(f"clamp step to {vstep} mV")
+    proto = {"kind": gwcapi.PROTO_CLAMP, "hold": hold, "step": vstep, "t_on": t_on, "t_off": t_off}
+    b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, 21, None, proto)])
+    b.set_mode(frozen=False, digest=True)
+    b.step(steps)
+    m = OracleModel(n, seed=21)
+    m.sample_rest_state()
+    t = 0.0
+    for _ in range(steps):
+        m.set_clamp(True, vstep if t_on <= t < t_off else hold)
+        assert m.step(1e-3, 0.0) == 0
+        t += 1e-3
+    assert_globals_equal(b.get_globals()[0], m.globals())
+    c_g, h_g = b.digest(0)
+    c_o, h_o = m.digest()
+    assert np.array_equal(c_g, c_o) and np.array_equal(h_g, h_o)
+    s = b.get_cru_state(0)
+    assert_states_equal(s, m.state())
+    b.close()
+    # the depolarising step opened L-type channels and released calcium
+    assert int(c_o.sum()) > n * 20
+
+
+def _oracle_globals_from_binding(gs):
+    o = GwGlobals()
+    for k in ("V", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN", "CaHTRPN", "m", "h", "j", "xKs"):
+        setattr(o, k, getattr(gs, k))
+    o.Kr[:] = list(np.asarray(gs.XKr).ravel())
+    o.Kv14[:] = list(np.asarray(gs.XKv14).ravel())
+    o.Kv43[:] = list(np.asarray(gs.XKv43).ravel())
+    return o
+
+
+def _stimulus_table(fn, steps, dt=1e-3):
+    out = np.zeros(steps)
+    t = 0.0
+    for k in range(steps):
+        out[k] = fn(t)
+        t += dt
+    return out
+
+
+RECORD_FIELDS = ("t", "V", "m", "h", "j", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN", "CaHTRPN", "xKs",
+                 "CaJSR", "CaSS", "LCC", "LCC_inactivation", "RyR", "ClCh")
+
+
+def test_pybind_run_matches_oracle(gw):
+    n, steps, rec, seed = 60, 1500, 300, 5
+    say("pybind run")
+    stim = lambda t: 100.0 if t < 1.0 else 0.0  # noqa: E731
+    p = gw.GWParameters()
+    c = gw.rest_state(n, seed)
+    gs = gw.GWGlobalState()
+    out = gw.run(p, n, 1e-3, steps, stim, rec, c, gs, PRNG="xoshiro256**", seed=seed)
+    m = OracleModel(n, seed=seed)
+    m.set_state(globals_=_oracle_globals_from_binding(gs), CaSS=c.CaSS, CaJSR=c.CaJSR, LCC=c.LCC,
+                LCC_inactivation=c.LCC_inactivation, RyR=c.RyR, ClCh=c.ClCh)
+    ref = m.run(1e-3, steps, _stimulus_table(stim, steps), record_every=rec)
+    assert out.V.shape == (steps // rec,)
+    for k in RECORD_FIELDS:
+        assert np.array_equal(np.asarray(getattr(out, k)), ref[k]), k
+    assert np.array_equal(np.asarray(out.XKr), ref["XKr"]) and np.array_equal(np.asarray(out.XKv14), ref["XKv14"])
+
+
+def test_gwmodel_simulate_matches_oracle(pkg):
+    n, steps, rec, seed = 40, 1200, 400, 8
+    say("GWModel.simulate")
+    params = pkg.GWParameters(NCaRU_sim=n)
+    stim = lambda t: 100.0 if t < 1.0 else 0.0  # noqa: E731
+    model = pkg.GWModel(parameters=params, stimulus_fn=stim)
+    sol = model.simulate(1e-3, steps, record_every=rec, PRNG="xoshiro256++", seed=seed)
+    import build.GreensteinWinslow as gw
+
+    c0 = gw.GWCRUState(n)
+    m = OracleModel(n, seed=seed, prng=1)
+    m.set_state(globals_=_oracle_globals_from_binding(gw.GWGlobalState()), CaSS=c0.CaSS, CaJSR=c0.CaJSR,
+                LCC=c0.LCC, LCC_inactivation=c0.LCC_inactivation, RyR=c0.RyR, ClCh=c0.ClCh)
+    ref = m.run(1e-3, steps, _stimulus_table(stim, steps), record_every=rec)
+    for k in RECORD_FIELDS:
+        assert np.array_equal(np.asarray(getattr(sol, k)), ref[k]), k
+    # post-hoc ICaL (src/model.py:414-449) from the recorded state
+    ical = sol.ICaL
+    assert ical.shape == (steps // rec,)
+    assert np.array_equal(ical, pkg.ICaL(ref["V"], ref["LCC"], ref["LCC_inactivation"], ref["CaSS"], params))
+    assert float(sol.V[-1]) > -40.0
+
+
+def test_state_round_trip_and_restart(gwcapi, gpu_ctx):
+    """get_cru_state / set_cru_state round-trip exactly, and a run split in two
+    calls equals one call (every call ends with all steps closed)."""
+    n = 300
+    say("state round trip")
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    a = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, 4, None, pace)])
+    a.set_mode(frozen=False, digest=True)
+    a.step(1500)
+    b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, 4, None, pace)])
+    b.set_mode(frozen=False, digest=True)
+    for k in (1, 2, 7, 490, 1000):
+        b.step(k)
+    assert_globals_equal(a.get_globals()[0], b.get_globals()[0])
+    assert_states_equal(a.get_cru_state(0), b.get_cru_state(0))
+    s = a.get_cru_state(0)
+    a.set_cru_state(0, s)
+    assert_states_equal(a.get_cru_state(0), s, STATE_KEYS)
+    a.close()
+    b.close()
