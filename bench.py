@@ -1,23 +1,28 @@
 #!/usr/bin/env python3
 """Throughput of the Greenstein-Winslow coupled integrator on one MI355X.
 
-Workload: 64 independent seeds (64 cells) of a 12,500-CRU cell, plain FP64,
-1 Hz pacing with a +100 pA/pF, 1 ms stimulus (the reference's sign: positive is
+Workload: 64 independent seeds (64 cells) of a 12,500-CRU cell, 1 Hz pacing
+with a +100 pA/pF, 1 ms stimulus (the reference's sign: positive is
 depolarising), dt = 1e-3 ms, starting from the seeded rest state. One bench
 step is one Euler step of the whole batch (every cell: global ODE + SSA of all
 its CRUs). W warmup steps run untimed first, then exactly K steps are timed
-with HIP events on the library's stream.
+with HIP events on the library's stream. Both forms run on the same batch:
+the batched FP32-local form (CRU-local arithmetic in FP32, global ODE and flux
+sums in FP64) and the plain FP64 form (bit-exact with the reference's
+algorithm).
 
-  value        CRU*ms simulated per second over the whole batch
+  value        CRU*ms simulated per second over the whole batch, FP32-local
+               form (the plain FP64 form is under "fp64")
   cell_beats   cells * (simulated ms / 1000 ms per beat) per second
-  roofline     for the SSA kernel k_ssa: algorithmic FP64 flops per launch
+  roofline     for the step kernel k_step: algorithmic flops per launch
                (F_ITER per SSA iteration x iterations counted in-kernel) and
-               algorithmic state bytes per launch (STATE_BYTES per CRU-step),
-               each divided by the kernel's average duration from HIP events
-               recorded by every SSA dispatch of the timed region; against the
-               FP64 vector peak and HBM. `traffic` is the measured HBM bytes per
-               launch from profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and
-               WRITE_SIZE, calibrated on a copy of known size) when present.
+               algorithmic state bytes per launch (bytes per CRU-step x CRUs),
+               each divided by the kernel's average duration from the start /
+               end events every dispatch of the timed region records; against
+               the vector peak of the form's precision and HBM. `traffic` is
+               the measured HBM bytes per launch from
+               profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and WRITE_SIZE,
+               calibrated on a copy of known size) when present.
   cpu_baseline the CPU restatement built with the reference's release flags,
                OpenMP over CRUs on all host cores, on a bounded sample
 """
@@ -42,7 +47,9 @@ PKG = os.path.join(ROOT, "gw-model_amd")
 # dt - t 1, CaSS 4 x 15, CaJSR 13.
 F_ITER = 331
 STATE_BYTES = 272       # bytes read + written per CRU-step by the FP64 SSA kernel: 136 in, 136 out
+STATE_BYTES_F32 = 200   # the same for the FP32-local kernel: 100 in, 100 out
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -57,9 +64,10 @@ def pacing(capi):
     return {"kind": capi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
 
 
-def gpu_bench(capi, cells, crus, steps, warmup, seed0=42):
+def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0):
     ctx = capi.Context(0)
-    b = capi.Batch(ctx, [capi.make_cell(crus, seed0 + c, None, pacing(capi)) for c in range(cells)])
+    b = capi.Batch(ctx, [capi.make_cell(crus, seed0 + c, None, pacing(capi)) for c in range(cells)],
+                   precision=precision)
     b.set_mode(frozen=False, digest=False)
     if warmup > 0:
         b.step(warmup)
@@ -73,8 +81,8 @@ def gpu_bench(capi, cells, crus, steps, warmup, seed0=42):
     return ms, ssa_ms, launches, iters, g, obs
 
 
-def measured_traffic(total_crus):
-    """HBM bytes per k_ssa launch from the committed PMC summary, scaled to this batch."""
+def measured_traffic(total_crus, form):
+    """HBM bytes per SSA launch from the committed PMC summary, scaled to this batch."""
     path = os.path.join(ROOT, "profiles", "k_ssa_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -84,7 +92,7 @@ def measured_traffic(total_crus):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_cru_step"]) * total_crus
+        return float(vals[f"hbm_bytes_per_cru_step_{form}"]) * total_crus
     except (KeyError, ValueError):
         return None
 
@@ -131,6 +139,46 @@ def copy_calibration(nbytes, reps=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def measure(capi, precision, args, total_crus):
+    """One form (FP64 or FP32-local) on the bench batch: throughput and the SSA kernel's roofline."""
+    f32 = precision == capi.PREC_FP32_LOCAL
+    ms, ssa_ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup,
+                                                    precision=precision)
+    sim_ms = args.steps * 1e-3
+    sec = ms / 1e3
+    per_launch_s = ssa_ms / 1e3 / launches
+    flops_per_launch = F_ITER * iters / launches
+    achieved_tflops = flops_per_launch / per_launch_s / 1e12
+    bytes_per_launch = (STATE_BYTES_F32 if f32 else STATE_BYTES) * total_crus
+    achieved_gbs = bytes_per_launch / per_launch_s / 1e9
+    peak_f = PEAK_FP32_TFLOPS if f32 else PEAK_FP64_TFLOPS
+    frac_f = achieved_tflops / peak_f
+    frac_b = achieved_gbs / PEAK_HBM_GBS
+    valu = frac_f >= frac_b
+    return {
+        "value": total_crus * sim_ms / sec,
+        "ms_per_step": ms / args.steps,
+        "cell_beats_per_s": args.cells * (sim_ms / 1000.0) / sec,
+        "iterations_per_cru_step": iters / (launches * total_crus),
+        "mean_V_end": float(np.mean([x.V for x in g])),
+        "roofline": {
+            "bound": "valu" if valu else "hbm",
+            "achieved": achieved_tflops if valu else achieved_gbs,
+            "peak": peak_f if valu else PEAK_HBM_GBS,
+            "unit": "TFLOP/s" if valu else "GB/s",
+            "frac": max(frac_f, frac_b),
+            "traffic": measured_traffic(total_crus, "fp32" if f32 else "fp64"),
+            "kernel": f"k_step<0,false,{'float' if f32 else 'double'}>",
+            "avg_launch_us": per_launch_s * 1e6,
+            "ssa_share_of_step": (ssa_ms / ms) if ms > 0 else None,
+            "valu": {"achieved_tflops": achieved_tflops, "peak": peak_f, "frac": frac_f,
+                     "flops_per_launch": flops_per_launch, "f_iter": F_ITER},
+            "hbm": {"achieved_gbs": achieved_gbs, "peak": PEAK_HBM_GBS, "frac": frac_b,
+                    "bytes_per_launch": bytes_per_launch},
+        },
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +189,8 @@ def main(argv=None):
     ap.add_argument("--cpu-crus", type=int, default=12500)
     ap.add_argument("--cpu-steps", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
+                    help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -153,61 +203,46 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
-    ms, ssa_ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup)
     total_crus = args.cells * args.crus
-    sim_ms = args.steps * 1e-3
-    sec = ms / 1e3
-    value = total_crus * sim_ms / sec
-    cell_beats = args.cells * (sim_ms / 1000.0) / sec
-    per_launch_s = ssa_ms / 1e3 / launches
-    flops_per_launch = F_ITER * iters / launches
-    achieved_tflops = flops_per_launch / per_launch_s / 1e12
-    bytes_per_launch = STATE_BYTES * total_crus
-    achieved_gbs = bytes_per_launch / per_launch_s / 1e9
-    frac_f = achieved_tflops / PEAK_FP64_TFLOPS
-    frac_b = achieved_gbs / PEAK_HBM_GBS
+    forms = {"fp32": capi.PREC_FP32_LOCAL, "fp64": capi.PREC_FP64}
+    chosen = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    res = {k: measure(capi, forms[k], args, total_crus) for k in chosen}
+    head = res[chosen[0]]
     line = {
         "metric": "CRU\u00b7ms simulated/s and cell-beats/s, batched FP32-local vs plain FP64, 1 GPU",
-        "value": value,
+        "value": head["value"],
         "unit": "CRU\u00b7ms/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-local/f64-global" if chosen[0] == "fp32" else "f64",
         "data": "synthetic: seeded rest state, 1 Hz pacing",
-        "config": {"workload": f"plain FP64, {args.cells} cells (independent seeds) x {args.crus} CRUs, "
-                               f"1 Hz pacing, dt 1e-3 ms, timed window t = {args.warmup * 1e-3:g}.."
-                               f"{(args.warmup + args.steps) * 1e-3:g} ms of beat 1",
-                   "cells": args.cells, "crus_per_cell": args.crus, "dt_ms": 1e-3, "precision": "fp64"},
-        "cell_beats_per_s": cell_beats,
-        "iterations_per_cru_step": iters / (launches * total_crus),
-        "roofline": {
-            "bound": "valu" if frac_f >= frac_b else "hbm",
-            "achieved": achieved_tflops if frac_f >= frac_b else achieved_gbs,
-            "peak": PEAK_FP64_TFLOPS if frac_f >= frac_b else PEAK_HBM_GBS,
-            "unit": "TFLOP/s" if frac_f >= frac_b else "GB/s",
-            "frac": max(frac_f, frac_b),
-            "traffic": measured_traffic(total_crus),
-            "kernel": "k_ssa<0,false>",
-            "avg_launch_us": per_launch_s * 1e6,
-            "ssa_share_of_step": (ssa_ms / ms) if ms > 0 else None,
-            "fp64": {"achieved_tflops": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "frac": frac_f,
-                     "flops_per_launch": flops_per_launch, "f_iter": F_ITER},
-            "hbm": {"achieved_gbs": achieved_gbs, "peak": PEAK_HBM_GBS, "frac": frac_b,
-                    "bytes_per_launch": bytes_per_launch},
-        },
-        "mean_V_end": float(np.mean([x.V for x in g])),
+        "config": {"workload": f"{args.cells} cells (independent seeds) x {args.crus} CRUs, 1 Hz pacing, "
+                               f"dt 1e-3 ms, timed window t = {args.warmup * 1e-3:g}.."
+                               f"{(args.warmup + args.steps) * 1e-3:g} ms of beat 1; "
+                               + ("batched FP32-local, plain FP64 beside it" if len(chosen) == 2 else chosen[0]),
+                   "cells": args.cells, "crus_per_cell": args.crus, "dt_ms": 1e-3, "precision": chosen[0]},
+        "cell_beats_per_s": head["cell_beats_per_s"],
+        "iterations_per_cru_step": head["iterations_per_cru_step"],
+        "roofline": head["roofline"],
     }
+    for k in chosen[1:]:
+        line[k] = {x: res[k][x] for x in ("value", "ms_per_step", "cell_beats_per_s", "iterations_per_cru_step",
+                                          "roofline")}
+    if len(chosen) == 2:
+        line["speedup_fp32_local_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    value = head["value"]
     if not args.no_cpu and rank == 0:
         v, threads, secs = cpu_baseline(args.cpu_crus, args.cpu_steps)
         line["cpu_baseline"] = {"value": v, "unit": "CRU\u00b7ms/s", "cores": threads, "kind": "port",
                                 "sample": f"1 cell x {args.cpu_crus} CRUs x {args.cpu_steps} steps "
                                           f"(t = 0..{args.cpu_steps * 1e-3:g} ms of beat 1), {secs:.1f} s",
-                                "ratio_gpu_over_cpu": value / v}
+                                "ratio_gpu_over_cpu": value / v,
+                                "ratio_fp64_gpu_over_cpu": res["fp64"]["value"] / v if "fp64" in res else None}
     if rank == 0:
         print(json.dumps(line), flush=True)
     return 0

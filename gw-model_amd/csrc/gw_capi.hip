@@ -116,7 +116,10 @@ __device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, do
     d.Istim = Istim;
     d.g = g;
     d.dg = dg;
-    if (kslot >= 0) step_consts(d.k[kslot], p, c, g.V, expm, g.Cai, g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+    if (kslot >= 0) {
+        step_consts(d.k[kslot], p, c, g.V, expm, g.Cai, g.CaNSR, static_cast<uint32_t>(d.step & 0xffff));
+        step_consts_f32(d.kf[kslot], d.k[kslot]);
+    }
 }
 
 // SSA constants of the step after d.step, into d.k[(d.step + 1) & 1]: the
@@ -134,7 +137,9 @@ __device__ void lookahead_consts(const DevCell &C, CellDyn &d, double dt, bool f
     if (d.clamp) V = protocol_vclamp(C, d.t + dt);
     const double VF_RT = V * C.c.F_RT;
     const double expm = gw::exp(-VF_RT);
-    step_consts(d.k[(d.step + 1) & 1], C.p, C.c, V, expm, Cai, CaNSR, static_cast<uint32_t>((d.step + 1) & 0xffff));
+    const int slot = static_cast<int>((d.step + 1) & 1);
+    step_consts(d.k[slot], C.p, C.c, V, expm, Cai, CaNSR, static_cast<uint32_t>((d.step + 1) & 0xffff));
+    step_consts_f32(d.kf[slot], d.k[slot]);
 }
 
 __device__ void sum_block_partials(const Partial *parts, int64_t first, int n, Partial &tot)
@@ -172,6 +177,20 @@ __device__ void partial_add(Partial &a, const Partial &b)
     a.err |= b.err;
 }
 
+// Exact tree reduction of NS partials in LDS: the `writer` threads store
+// theirs at red[slot] (together they write every slot 0..NS-1), then the
+// first NS/2 threads fold the tree. All threads of the block call it.
+template <int NS>
+__device__ void reduce_slots(Partial *red, const Partial &mine, int slot, bool writer)
+{
+    if (writer) red[slot] = mine;
+    __syncthreads();
+    for (int s = NS / 2; s > 0; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) partial_add(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+}
+
 template <int NT>
 __device__ void block_reduce(Partial *red, Partial mine)
 {
@@ -206,11 +225,11 @@ __global__ void k_seed(KArgs a)
             a.dig[slot * 2] = 0;
             a.dig[slot * 2 + 1] = 0xcbf29ce484222325ULL;
         }
-        if (i >= C.n_cru) a.CaJSR[slot] = 0.0;
+        if (i >= C.n_cru) st_cajsr(a, slot, 0.0);
     }
     if (i >= C.n_cru) {
-        a.CaSS[slot * 4 + j] = 0.0;
-        a.JLCC[slot * 4 + j] = 0.0;
+        st_cass(a, slot * 4 + j, 0.0);
+        st_jlcc(a, slot * 4 + j, 0.0);
         a.sub[slot * 4 + j] = sub_set_lcc(0, 1);
     }
 }
@@ -230,11 +249,11 @@ __global__ void k_sample_rest(KArgs a)
     uint64_t sub[4];
     sample_rest_cru<PRNG>(C.seed, i, ca, jsr, sub);
     for (int j = 0; j < 4; ++j) {
-        a.CaSS[slot * 4 + j] = ca[j];
-        a.JLCC[slot * 4 + j] = 0.0;
+        st_cass(a, slot * 4 + j, ca[j]);
+        st_jlcc(a, slot * 4 + j, 0.0);
         a.sub[slot * 4 + j] = sub[j];
     }
-    a.CaJSR[slot] = jsr;
+    st_cajsr(a, slot, jsr);
 }
 
 // Step-0 preparation of every cell (one block per cell): the flux sums the
@@ -255,7 +274,7 @@ __global__ void k_init_cells(KArgs a)
     for (int64_t i = threadIdx.x; i < C.n_cru; i += kBlock) {
         const int64_t slot = C.slot0 + i;
         double jl[4];
-        for (int j = 0; j < 4; ++j) jl[j] = a.JLCC[slot * 4 + j];
+        for (int j = 0; j < 4; ++j) jl[j] = ld_jlcc(a, slot * 4 + j);
         mine.err |= fx_add(mine.jlcc, jl[0] + jl[1] + jl[2] + jl[3]);
         mine.err |= fx_add(mine.jxfer, X4);
         mine.err |= fx_add(mine.jtr, Y);
@@ -283,11 +302,12 @@ __global__ void k_init_cells(KArgs a)
 
 // Copy a step's LCC rate table into LDS (it is read with a per-lane index;
 // the cell's other constants are wave-uniform and are read as scalars).
-__device__ __forceinline__ void load_lcc_table(const StepConsts &K, LccTable &T)
+template <class KT, class TT>
+__device__ __forceinline__ void load_lcc_table(const KT &K, TT &T)
 {
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&K.lcc);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(&T);
-    for (int w = threadIdx.x; w < static_cast<int>(sizeof(LccTable) / 8); w += blockDim.x) dst[w] = src[w];
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&K.lcc);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&T);
+    for (int w = threadIdx.x; w < static_cast<int>(sizeof(TT) / 4); w += blockDim.x) dst[w] = src[w];
 }
 
 __device__ __forceinline__ void partial_from(Partial &m, const QuadOut &o, uint64_t iters)
@@ -306,18 +326,36 @@ __device__ __forceinline__ void partial_from(Partial &m, const QuadOut &o, uint6
 // the 64 CRUs of SSA block b, with the constants of step step0 (nsteps > 1
 // only in frozen-globals replay, where they do not change). The block's flux
 // partial of the last step goes to partials[step0 & 1][b]; only the quad
-// leaders contribute.
-template <int PRNG, bool DIGEST>
-__device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTable &T, Partial *red)
+// leaders contribute. R is the CRU-local arithmetic (double, or float for the
+// FP32-local variant).
+template <class R> struct LocalState;
+template <> struct LocalState<double> {
+    static __device__ __forceinline__ double *cass(const KArgs &a) { return a.CaSS; }
+    static __device__ __forceinline__ double *jlcc(const KArgs &a) { return a.JLCC; }
+    static __device__ __forceinline__ double *cajsr(const KArgs &a) { return a.CaJSR; }
+    static __device__ __forceinline__ const SsaParams &params(const DevCell &C) { return C.sp; }
+    static __device__ __forceinline__ const StepConsts &consts(const CellDyn &D, int ks) { return D.k[ks]; }
+};
+template <> struct LocalState<float> {
+    static __device__ __forceinline__ float *cass(const KArgs &a) { return a.CaSSf; }
+    static __device__ __forceinline__ float *jlcc(const KArgs &a) { return a.JLCCf; }
+    static __device__ __forceinline__ float *cajsr(const KArgs &a) { return a.CaJSRf; }
+    static __device__ __forceinline__ const SsaParamsF &params(const DevCell &C) { return C.spf; }
+    static __device__ __forceinline__ const StepConstsF &consts(const CellDyn &D, int ks) { return D.kf[ks]; }
+};
+
+template <int PRNG, bool DIGEST, class R>
+__device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTableT<R> &T, Partial *red)
 {
+    using LS = LocalState<R>;
     const int tid = threadIdx.x;
     const int j = tid & 3;
     const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
     const int ks = static_cast<int>(step0 & 1);
     const DevCell &C = a.cells[cell];
     const CellDyn &D = a.dyn[cell];
-    const SsaParams &sp = C.sp;
-    const StepConsts &kk = D.k[ks];
+    const auto &sp = LS::params(C);
+    const auto &kk = LS::consts(D, ks);
     load_lcc_table(kk, T);
     __syncthreads();
     const int64_t i = (b - C.blk0) * kCruPerBlock + (tid >> 2);
@@ -325,12 +363,15 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     Partial mine;
     partial_zero(mine);
     if (active) {
+        R *const CaSS = LS::cass(a);
+        R *const JLCC = LS::jlcc(a);
+        R *const CaJSR = LS::cajsr(a);
         const int64_t slot = C.slot0 + i;
-        double ca = a.CaSS[slot * 4 + j];
-        double jl = a.JLCC[slot * 4 + j];
+        R ca = CaSS[slot * 4 + j];
+        R jl = JLCC[slot * 4 + j];
         uint64_t w = a.sub[slot * 4 + j];
-        CruQ q;
-        q.jsr = a.CaJSR[slot];
+        CruQT<R> q;
+        q.jsr = CaJSR[slot];
         q.r0 = a.rng[slot * 4];
         q.r1 = a.rng[slot * 4 + 1];
         q.r2 = a.rng[slot * 4 + 2];
@@ -342,16 +383,17 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
         }
         QuadOut o{};
         uint64_t iters = 0;
+        const R dt = static_cast<R>(a.dt);
         for (int64_t s = 0; s < nsteps; ++s) {
-            o = ssa_step_quad<PRNG, DIGEST>(ca, jl, w, q, sp, kk, T, a.dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff), j);
+            o = ssa_step_quad<PRNG, DIGEST>(ca, jl, w, q, sp, kk, T, dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff), j);
             iters += o.iters;
             if (o.err) break;
         }
-        a.CaSS[slot * 4 + j] = ca;
-        a.JLCC[slot * 4 + j] = jl;
+        CaSS[slot * 4 + j] = ca;
+        JLCC[slot * 4 + j] = jl;
         a.sub[slot * 4 + j] = w;
         if (j == 0) {
-            a.CaJSR[slot] = q.jsr;
+            CaJSR[slot] = q.jsr;
             a.rng[slot * 4] = q.r0;
             a.rng[slot * 4 + 1] = q.r1;
             a.rng[slot * 4 + 2] = q.r2;
@@ -363,7 +405,7 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
             partial_from(mine, o, iters);
         }
     }
-    block_reduce<kBlock>(red, mine);
+    reduce_slots<kCruPerBlock>(red, mine, tid >> 2, j == 0);
     if (tid == 0) a.partials[ks * a.nblocks + b] = red[0];
 }
 
@@ -372,16 +414,18 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
 // derivatives of step n+1, observables and trace, and the SSA constants of
 // step n+2 (those of step n+1 were formed one close earlier and are in use by
 // the SSA blocks running beside this one).
-template <int NT>
+template <int NS>
 __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *red)
 {
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
     const int ks = static_cast<int>(D.step & 1);
+    const int tid = threadIdx.x;
     Partial mine;
     partial_zero(mine);
-    for (int q = threadIdx.x; q < C.nblk; q += NT) partial_add(mine, a.partials[ks * a.nblocks + C.blk0 + q]);
-    block_reduce<NT>(red, mine);
+    if (tid < NS)
+        for (int q = tid; q < C.nblk; q += NS) partial_add(mine, a.partials[ks * a.nblocks + C.blk0 + q]);
+    reduce_slots<NS>(red, mine, tid, tid < NS);
     if (threadIdx.x != 0) return;
     const Partial &tot = red[0];
     if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
@@ -422,26 +466,31 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *re
 // share no data: the SSA reads the constants of its step (formed by the close
 // one launch earlier) and writes the other partials buffer. ssa_step < 0
 // launches the closes alone (the last close of a sequence).
-template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock, 4) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
+// Resident waves per SIMD the SSA kernels are compiled for: FP64 needs about
+// 128 VGPRs; the FP32-local SSA fits 80 (the single-lane close path then
+// spills to scratch, which it can afford).
+template <class R> constexpr int kWavesPerSimd = sizeof(R) == 4 ? 6 : 4;
+
+template <int PRNG, bool DIGEST, class R>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
 {
-    __shared__ LccTable T;
-    __shared__ Partial red[kBlock];
+    __shared__ LccTableT<R> T;
+    __shared__ Partial red[kCruPerBlock];
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kBlock>(a, bid, red);
+        close_cell<kCruPerBlock>(a, bid, red);
         return;
     }
-    ssa_block<PRNG, DIGEST>(a, bid - nclose, ssa_step, 1, T, red);
+    ssa_block<PRNG, DIGEST, R>(a, bid - nclose, ssa_step, 1, T, red);
 }
 
 // Frozen-globals replay: nsteps SSA steps in one launch.
-template <int PRNG, bool DIGEST>
-__global__ __launch_bounds__(kBlock, 4) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
+template <int PRNG, bool DIGEST, class R>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
 {
-    __shared__ LccTable T;
-    __shared__ Partial red[kBlock];
-    ssa_block<PRNG, DIGEST>(a, blockIdx.x, step0, nsteps, T, red);
+    __shared__ LccTableT<R> T;
+    __shared__ Partial red[kCruPerBlock];
+    ssa_block<PRNG, DIGEST, R>(a, blockIdx.x, step0, nsteps, T, red);
 }
 
 // Close a frozen replay of nsteps: sums of the last step, clock advanced as
@@ -475,13 +524,13 @@ __global__ void k_put_state(KArgs a, int cell, StateView v, int has_jlcc)
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
     for (int j = 0; j < 4; ++j) {
-        a.CaSS[slot * 4 + j] = v.CaSS[i * 4 + j];
-        a.JLCC[slot * 4 + j] = has_jlcc ? v.JLCC[i * 4 + j] : 0.0;
+        st_cass(a, slot * 4 + j, v.CaSS[i * 4 + j]);
+        st_jlcc(a, slot * 4 + j, has_jlcc ? v.JLCC[i * 4 + j] : 0.0);
         int n6[6];
         for (int k = 0; k < 6; ++k) n6[k] = v.RyR[(i * 4 + j) * 6 + k];
         a.sub[slot * 4 + j] = sub_pack(v.LCC[i * 4 + j], v.LCCi[i * 4 + j], v.ClCh[i * 4 + j], n6);
     }
-    a.CaJSR[slot] = v.CaJSR[i];
+    st_cajsr(a, slot, v.CaJSR[i]);
 }
 
 __global__ void k_get_state(KArgs a, int cell, StateView v)
@@ -491,15 +540,15 @@ __global__ void k_get_state(KArgs a, int cell, StateView v)
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
     for (int j = 0; j < 4; ++j) {
-        v.CaSS[i * 4 + j] = a.CaSS[slot * 4 + j];
-        v.JLCC[i * 4 + j] = a.JLCC[slot * 4 + j];
+        v.CaSS[i * 4 + j] = ld_cass(a, slot * 4 + j);
+        v.JLCC[i * 4 + j] = ld_jlcc(a, slot * 4 + j);
         const uint64_t w = a.sub[slot * 4 + j];
         v.LCC[i * 4 + j] = sub_lcc(w);
         v.LCCi[i * 4 + j] = sub_y(w);
         v.ClCh[i * 4 + j] = sub_cl(w);
         for (int k = 0; k < 6; ++k) v.RyR[(i * 4 + j) * 6 + k] = ryr_get(w, k);
     }
-    v.CaJSR[i] = a.CaJSR[slot];
+    v.CaJSR[i] = ld_cajsr(a, slot);
 }
 
 __global__ void k_reset_obs(CellDyn *dyn, int ncells)
@@ -636,10 +685,12 @@ int prepare(gw_batch *B, char *err, size_t errlen)
 template <int PRNG>
 void launch_frozen(gw_batch *B, int64_t step0, int64_t n)
 {
-    if (B->digest)
-        hipLaunchKernelGGL((k_ssa<PRNG, true>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+    if (B->precision == GW_PREC_FP32_LOCAL)
+        hipLaunchKernelGGL((k_ssa<PRNG, false, float>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+    else if (B->digest)
+        hipLaunchKernelGGL((k_ssa<PRNG, true, double>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
     else
-        hipLaunchKernelGGL((k_ssa<PRNG, false>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
+        hipLaunchKernelGGL((k_ssa<PRNG, false, double>), dim3(B->nblocks), dim3(kBlock), 0, B->ctx->stream, B->a, step0, n);
     hipLaunchKernelGGL(k_frozen_close, dim3((B->ncells + 63) / 64), dim3(64), 0, B->ctx->stream, B->a, step0, n);
 }
 
@@ -662,10 +713,12 @@ template <int PRNG>
 void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
 {
     const unsigned grid = static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
-    if (B->digest)
-        hipExtLaunchKernelGGL((k_step<PRNG, true>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+    if (B->precision == GW_PREC_FP32_LOCAL)
+        hipExtLaunchKernelGGL((k_step<PRNG, false, float>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+    else if (B->digest)
+        hipExtLaunchKernelGGL((k_step<PRNG, true, double>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
     else
-        hipExtLaunchKernelGGL((k_step<PRNG, false>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+        hipExtLaunchKernelGGL((k_step<PRNG, false, double>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
 }
 
 void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
@@ -895,7 +948,7 @@ int gw_batch_destroy(gw_batch *B)
     live.erase(std::remove(live.begin(), live.end(), B), live.end());
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
-    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
+    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.CaSSf, B->a.CaJSRf, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
                     B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -921,8 +974,8 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         set_err(err, errlen, "unknown PRNG id %d", prng_id);
         return GW_E_INVALID_ARG;
     }
-    if (precision != GW_PREC_FP64) {
-        set_err(err, errlen, "precision %d is not available in this build", precision);
+    if (precision != GW_PREC_FP64 && precision != GW_PREC_FP32_LOCAL) {
+        set_err(err, errlen, "unknown precision %d", precision);
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(ctx->device));
@@ -957,6 +1010,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         C.p = d.params;
         cell_consts_init(C.c, C.p, d.n_cru);
         ssa_params_init(C.sp, C.p, C.c);
+        ssa_params_f32(C.spf, C.sp);
         C.proto = d.protocol;
         C.seed = d.seed;
         C.n_cru = d.n_cru;
@@ -972,10 +1026,14 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     const int64_t N = B->ntot;
     KArgs &a = B->a;
     int rc = GW_OK;
-    if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
-        (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)) || (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
+    const bool f32 = precision == GW_PREC_FP32_LOCAL;
+    if ((!f32 && ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
+                  (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)))) ||
+        (f32 && ((rc = dmalloc(&a.CaSSf, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSRf, N, err, errlen)) ||
+                 (rc = dmalloc(&a.JLCCf, 4 * N, err, errlen)))) ||
+        (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
-        (rc = dmalloc(&a.dig, 2 * N, err, errlen)) || (rc = dmalloc(&a.partials, 2 * B->nblocks, err, errlen)) ||
+        (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->nblocks, err, errlen)) ||
         (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
         (rc = dmalloc(&B->d_block_cell, B->nblocks, err, errlen))) {
@@ -994,6 +1052,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     a.trace_cap = 0;
     a.trace = nullptr;
     a.prng = prng_id;
+    a.precision = precision;
     // per-cell stimulus tables are attached later; globals start at g0 or the defaults
     std::vector<CellDyn> dyn(ncells);
     B->g0.resize(ncells);

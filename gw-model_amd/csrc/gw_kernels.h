@@ -14,6 +14,7 @@ struct DevCell {
     gw_params_t p;
     CellConsts c;
     SsaParams sp;
+    SsaParamsF spf;        // the same, rounded to FP32 (FP32-local batches)
     gw_protocol_t proto;
     const double *istim;   // per-step stimulus table (GW_PROTO_TABLE) or null
     int64_t istim_len;
@@ -29,6 +30,7 @@ struct CellDyn {
     gw_globals_t g;      // globals at the start of the current step
     gw_globals_t dg;     // increments formed before the current step's SSA
     StepConsts k[2];     // SSA constants of steps with even / odd index
+    StepConstsF kf[2];   // the same, rounded to FP32 (FP32-local batches)
     double t;            // accumulated time at the start of the current step
     double Istim;        // stimulus of the current step
     double sums[6];      // JLCC, Jxfer, Jtr, ClCh, open RyR, Jrel of the last SSA
@@ -52,9 +54,12 @@ struct KArgs {
     const DevCell *cells;
     CellDyn *dyn;
     const int32_t *block_cell;
-    double *CaSS;      // [ntot][4]
+    double *CaSS;      // [ntot][4]   (FP64 batches)
     double *CaJSR;     // [ntot]
     double *JLCC;      // [ntot][4]
+    float *CaSSf;      // [ntot][4]   (FP32-local batches)
+    float *CaJSRf;     // [ntot]
+    float *JLCCf;      // [ntot][4]
     uint64_t *sub;     // [ntot][4] packed subunit words (RyR counts, LCC, y, ClCh)
     uint64_t *rng;     // [ntot][4] engine words
     uint64_t *dig;     // [ntot][2] (count, hash) when digests are kept
@@ -69,6 +74,25 @@ struct KArgs {
     int32_t frozen;
     int32_t trace_every;
     int32_t prng;
+    int32_t precision;   // GW_PREC_FP64 or GW_PREC_FP32_LOCAL
+    int32_t _pad;
 };
+
+// CRU-local state of either precision, for the kernels outside the SSA.
+__device__ __forceinline__ double ld_cass(const KArgs &a, int64_t k) { return a.CaSSf ? static_cast<double>(a.CaSSf[k]) : a.CaSS[k]; }
+__device__ __forceinline__ double ld_jlcc(const KArgs &a, int64_t k) { return a.JLCCf ? static_cast<double>(a.JLCCf[k]) : a.JLCC[k]; }
+__device__ __forceinline__ double ld_cajsr(const KArgs &a, int64_t k) { return a.CaJSRf ? static_cast<double>(a.CaJSRf[k]) : a.CaJSR[k]; }
+__device__ __forceinline__ void st_cass(const KArgs &a, int64_t k, double v)
+{
+    if (a.CaSSf) a.CaSSf[k] = static_cast<float>(v); else a.CaSS[k] = v;
+}
+__device__ __forceinline__ void st_jlcc(const KArgs &a, int64_t k, double v)
+{
+    if (a.JLCCf) a.JLCCf[k] = static_cast<float>(v); else a.JLCC[k] = v;
+}
+__device__ __forceinline__ void st_cajsr(const KArgs &a, int64_t k, double v)
+{
+    if (a.CaJSRf) a.CaJSRf[k] = static_cast<float>(v); else a.CaJSR[k] = v;
+}
 
 }  // namespace gw

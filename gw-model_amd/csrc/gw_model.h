@@ -64,9 +64,12 @@ GW_HD void cell_consts_init(CellConsts &c, const gw_params_t &p, int nCRU)
 // are tabulated so that (r0 + r1) + r2 = lccP[s] + lccQ[s] * CaSS bit for bit;
 // R0, R1 hold r0, r1 with the CaSS-proportional slot set to zero (r2 is
 // never needed on its own: the transition scan stops before comparing it).
-struct LccTable {
-    double P[13], Q[13], R0[13], R1[13];
+template <class R>
+struct LccTableT {
+    R P[13], Q[13], R0[13], R1[13];
 };
+using LccTable = LccTableT<double>;
+using LccTableF = LccTableT<float>;
 
 struct StepConsts {
     double Cai, CaNSR;
@@ -125,6 +128,30 @@ GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c,
         if (s == 1) k.lcc.P[s] = r0[s];
         else if (s >= 2 && s <= 5) k.lcc.P[s] = r0[s] + r1[s];
         else k.lcc.P[s] = r0[s] + r1[s] + r2[s];
+    }
+}
+
+// The SSA constants of a step rounded to FP32, for the FP32-local SSA.
+struct StepConstsF {
+    float Cai, CaNSR;
+    float inact0, inact1;
+    float JLCC_mult, JLCC_exp;
+    LccTableF lcc;
+};
+
+GW_HD void step_consts_f32(StepConstsF &f, const StepConsts &k)
+{
+    f.Cai = static_cast<float>(k.Cai);
+    f.CaNSR = static_cast<float>(k.CaNSR);
+    f.inact0 = static_cast<float>(k.inact0);
+    f.inact1 = static_cast<float>(k.inact1);
+    f.JLCC_mult = static_cast<float>(k.JLCC_mult);
+    f.JLCC_exp = static_cast<float>(k.JLCC_exp);
+    for (int s = 0; s < 13; ++s) {
+        f.lcc.P[s] = static_cast<float>(k.lcc.P[s]);
+        f.lcc.Q[s] = static_cast<float>(k.lcc.Q[s]);
+        f.lcc.R0[s] = static_cast<float>(k.lcc.R0[s]);
+        f.lcc.R1[s] = static_cast<float>(k.lcc.R1[s]);
     }
 }
 

@@ -1,5 +1,5 @@
-// gw_ssa.h -- per-CRU Gillespie SSA over one global step, FP64, on a quad of
-// lanes (one lane per subunit).
+// gw_ssa.h -- per-CRU Gillespie SSA over one global step on a quad of lanes
+// (one lane per subunit), in FP64 or in the FP32-local variant.
 //
 // Restates SSA_single_CRU and its helpers (src/includes/SSA.tpp:10-310,
 // SSA.hpp:67-71, GW_utils.tpp:118-255, GW_utils.hpp:378-391) for gfx950:
@@ -15,8 +15,11 @@
 //     sampled, from the inputs the reference's stored rate arrays were formed
 //     from (CaSS before this iteration's update, RyR counts before any
 //     threshold re-split), so every comparison sees identical values;
-//   - all arithmetic keeps the reference's operation order (bit-exact with
-//     the CPU oracle when compiled with -ffp-contract=off).
+//   - all arithmetic keeps the reference's operation order (in FP64,
+//     bit-exact with the CPU oracle when compiled with -ffp-contract=off);
+//   - the FP32-local variant runs the same code on floats: CRU-local Ca,
+//     fluxes and propensities in FP32 with hardware reciprocal and log2, the
+//     integer channel states and the random streams unchanged.
 #pragma once
 
 #include "gw_model.h"
@@ -60,13 +63,17 @@ GW_HD int lcc_next(int s, int tr)
     return tr == 0 ? s - 6 : (tr == 1 ? s - 1 : s + 1);
 }
 
-// Per-cell parameters the SSA reads (a subset of gw_params_t and CellConsts).
-struct SsaParams {
-    double rRyR, rxfer, rtr, riss;
-    double KBSR, KBSL, KCSQN, BSR_const, BSL_const, CSQN_const, VSS_VJSR;
-    double k12, k21, k23, k32, k34, k43, k45, k54, k56, k65, k25, k52;
-    double kfClCh, kbClCh, Cao_scaled;
+// Per-cell parameters the SSA reads (a subset of gw_params_t and CellConsts),
+// in the SSA's local arithmetic type.
+template <class R>
+struct SsaParamsT {
+    R rRyR, rxfer, rtr, riss;
+    R KBSR, KBSL, KCSQN, BSR_const, BSL_const, CSQN_const, VSS_VJSR;
+    R k12, k21, k23, k32, k34, k43, k45, k54, k56, k65, k25, k52;
+    R kfClCh, kbClCh, Cao_scaled;
 };
+using SsaParams = SsaParamsT<double>;
+using SsaParamsF = SsaParamsT<float>;
 
 GW_HD void ssa_params_init(SsaParams &s, const gw_params_t &p, const CellConsts &c)
 {
@@ -78,41 +85,58 @@ GW_HD void ssa_params_init(SsaParams &s, const gw_params_t &p, const CellConsts 
     s.kfClCh = p.kfClCh; s.kbClCh = p.kbClCh; s.Cao_scaled = c.Cao_scaled;
 }
 
+GW_HD void ssa_params_f32(SsaParamsF &f, const SsaParams &s)
+{
+    const double *src = &s.rRyR;
+    float *dst = &f.rRyR;
+    for (int i = 0; i < static_cast<int>(sizeof(SsaParams) / sizeof(double)); ++i) dst[i] = static_cast<float>(src[i]);
+}
+
+// Local arithmetic of the SSA. FP64: IEEE division and the shared log, bit-exact
+// with the oracle. FP32 (the FP32-local variant): one hardware reciprocal
+// (v_rcp_f32, 1 ulp) times the numerator, and the hardware log2 (v_log_f32).
+__device__ __forceinline__ double ldiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float ldiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double lneglog(double u) { return -gw::log(u); }
+__device__ __forceinline__ float lneglog(float u) { return -(__builtin_amdgcn_logf(u) * 0.693147182f); }
+__device__ __forceinline__ float sq(float x) { return x * x; }
+
 // The 12 RyR propensities of one subunit (update_RyR_rates, GW_utils.tpp:222-255),
 // visited in storage order; `f(k, r)` receives each rate.
-template <class F>
-GW_HD void ryr_rates(const SsaParams &P, uint64_t w, double ca, F &&f)
+template <class R, class F>
+__device__ __forceinline__ void ryr_rates(const SsaParamsT<R> &P, uint64_t w, R ca, F &&f)
 {
-    const double C2 = ca * ca;
+    const R C2 = ca * ca;
     const int n1 = ryr_get(w, 0), n2 = ryr_get(w, 1), n3 = ryr_get(w, 2);
     const int n4 = ryr_get(w, 3), n5 = ryr_get(w, 4), n6 = ryr_get(w, 5);
-    const bool hi34 = ca > 3.685e-2;
-    const bool hi56 = ca > 1.15e-4;
-    double tau34 = 0.0, eq56 = 0.0;
-    if (hi34) tau34 = 1.0 / (P.k34 * C2 + P.k43);
-    if (hi56) eq56 = P.k65 / (P.k56 * C2 + P.k65);
-    f(0, static_cast<double>(n1) * P.k12 * C2);
-    f(1, static_cast<double>(n2) * P.k23 * C2);
-    f(2, static_cast<double>(n2) * P.k25 * C2);
-    f(3, hi34 ? 0.0 : static_cast<double>(n3) * P.k34 * C2);
-    f(4, hi34 ? static_cast<double>(n3 + n4) * P.k45 * P.k34 * C2 * tau34 : static_cast<double>(n4) * P.k45);
-    f(5, hi56 ? 0.0 : static_cast<double>(n5) * P.k56 * C2);
-    f(6, static_cast<double>(n2) * P.k21);
-    f(7, hi34 ? static_cast<double>(n3 + n4) * P.k32 * P.k43 * tau34 : static_cast<double>(n3) * P.k32);
-    f(8, hi34 ? 0.0 : static_cast<double>(n4) * P.k43);
-    f(9, hi56 ? static_cast<double>(n5 + n6) * P.k52 * eq56 : static_cast<double>(n5) * P.k52);
-    f(10, hi56 ? static_cast<double>(n5 + n6) * P.k54 * C2 * eq56 : static_cast<double>(n5) * P.k54 * C2);
-    f(11, hi56 ? 0.0 : static_cast<double>(n6) * P.k65);
+    const bool hi34 = ca > R(3.685e-2);
+    const bool hi56 = ca > R(1.15e-4);
+    R tau34 = R(0), eq56 = R(0);
+    if (hi34) tau34 = ldiv(R(1), P.k34 * C2 + P.k43);
+    if (hi56) eq56 = ldiv(P.k65, P.k56 * C2 + P.k65);
+    f(0, static_cast<R>(n1) * P.k12 * C2);
+    f(1, static_cast<R>(n2) * P.k23 * C2);
+    f(2, static_cast<R>(n2) * P.k25 * C2);
+    f(3, hi34 ? R(0) : static_cast<R>(n3) * P.k34 * C2);
+    f(4, hi34 ? static_cast<R>(n3 + n4) * P.k45 * P.k34 * C2 * tau34 : static_cast<R>(n4) * P.k45);
+    f(5, hi56 ? R(0) : static_cast<R>(n5) * P.k56 * C2);
+    f(6, static_cast<R>(n2) * P.k21);
+    f(7, hi34 ? static_cast<R>(n3 + n4) * P.k32 * P.k43 * tau34 : static_cast<R>(n3) * P.k32);
+    f(8, hi34 ? R(0) : static_cast<R>(n4) * P.k43);
+    f(9, hi56 ? static_cast<R>(n5 + n6) * P.k52 * eq56 : static_cast<R>(n5) * P.k52);
+    f(10, hi56 ? static_cast<R>(n5 + n6) * P.k54 * C2 * eq56 : static_cast<R>(n5) * P.k54 * C2);
+    f(11, hi56 ? R(0) : static_cast<R>(n6) * P.k65);
 }
 
 // Subunit total in the reference accumulation order (SSA.tpp:48-56):
 // 0 + LCC sum, + inactivation, + RyR[0..11], + ClCh.
-GW_HD double subunit_total(const SsaParams &P, const StepConsts &K, const LccTable &T, uint64_t w, double ca)
+template <class R, class KT>
+__device__ __forceinline__ R subunit_total(const SsaParamsT<R> &P, const KT &K, const LccTableT<R> &T, uint64_t w, R ca)
 {
     const int L = sub_lcc(w);
-    double s = T.P[L] + T.Q[L] * ca;
+    R s = T.P[L] + T.Q[L] * ca;
     s += sub_y(w) ? K.inact1 : K.inact0;
-    ryr_rates(P, w, ca, [&](int, double r) { s += r; });
+    ryr_rates(P, w, ca, [&](int, R r) { s += r; });
     s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
     return s;
 }
@@ -202,17 +226,21 @@ __device__ __forceinline__ uint64_t qmov64(uint64_t u)
            qmov32<CTRL>(static_cast<uint32_t>(u));
 }
 template <int CTRL>
-__device__ __forceinline__ double qmovd(double v) { return double_of(qmov64<CTRL>(bits_of(v))); }
+__device__ __forceinline__ double qmov(double v) { return double_of(qmov64<CTRL>(bits_of(v))); }
+template <int CTRL>
+__device__ __forceinline__ float qmov(float v) { return __builtin_bit_cast(float, qmov32<CTRL>(__builtin_bit_cast(uint32_t, v))); }
 template <int CTRL>
 __device__ __forceinline__ int qmovi(int v) { return static_cast<int>(qmov32<CTRL>(static_cast<uint32_t>(v))); }
 
-template <int K>
-__device__ __forceinline__ double qgetd(double v) { return qmovd<K * 0x55>(v); }   // lane K of the quad
+template <int K, class R>
+__device__ __forceinline__ R qget(R v) { return qmov<K * 0x55>(v); }          // lane K of the quad
 template <int K>
 __device__ __forceinline__ int qgeti(int v) { return qmovi<K * 0x55>(v); }
-__device__ __forceinline__ double qnext(double v) { return qmovd<0x39>(v); }      // lane (j + 1) & 3
-__device__ __forceinline__ double qprev(double v) { return qmovd<0x93>(v); }      // lane (j + 3) & 3
-__device__ __forceinline__ int qseli(int v, int k)                                // lane k (quad-uniform k)
+template <class R>
+__device__ __forceinline__ R qnext(R v) { return qmov<0x39>(v); }             // lane (j + 1) & 3
+template <class R>
+__device__ __forceinline__ R qprev(R v) { return qmov<0x93>(v); }             // lane (j + 3) & 3
+__device__ __forceinline__ int qseli(int v, int k)                             // lane k (quad-uniform k)
 {
     const int a = qgeti<0>(v), b = qgeti<1>(v), c = qgeti<2>(v), d = qgeti<3>(v);
     return k == 0 ? a : k == 1 ? b : k == 2 ? c : d;
@@ -222,14 +250,22 @@ __device__ __forceinline__ uint64_t qselu(uint64_t v, int k)
     const uint64_t a = qmov64<0x00>(v), b = qmov64<0x55>(v), c = qmov64<0xAA>(v), d = qmov64<0xFF>(v);
     return k == 0 ? a : k == 1 ? b : k == 2 ? c : d;
 }
-__device__ __forceinline__ double qsum4(double v) { return qgetd<0>(v) + qgetd<1>(v) + qgetd<2>(v) + qgetd<3>(v); }
+__device__ __forceinline__ double qsel(double v, int k) { return double_of(qselu(bits_of(v), k)); }
+__device__ __forceinline__ float qsel(float v, int k)
+{
+    return __builtin_bit_cast(float, static_cast<uint32_t>(qseli(static_cast<int>(__builtin_bit_cast(uint32_t, v)), k)));
+}
+template <class R>
+__device__ __forceinline__ R qsum4(R v) { return qget<0>(v) + qget<1>(v) + qget<2>(v) + qget<3>(v); }
 __device__ __forceinline__ int qsum4i(int v) { return qgeti<0>(v) + qgeti<1>(v) + qgeti<2>(v) + qgeti<3>(v); }
 
 // Replicated CRU-level state of a quad.
-struct CruQ {
-    double jsr;
+template <class R>
+struct CruQT {
+    R jsr;
     uint64_t r0, r1, r2, r3;
 };
+using CruQ = CruQT<double>;
 
 // Per-CRU outputs of one step for the reductions (identical in the 4 lanes).
 struct QuadOut {
@@ -243,55 +279,61 @@ struct QuadOut {
     int err;         // nonzero if the iteration cap was hit
 };
 
+// Uniform draws: FP64 uses the reference's 64-bit map; FP32 the top 24 bits.
 template <int PRNG>
-__device__ __forceinline__ double qdraw(CruQ &q) { return u01(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3)); }
+__device__ __forceinline__ double qdraw(CruQT<double> &q) { return u01(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3)); }
+template <int PRNG>
+__device__ __forceinline__ float qdraw(CruQT<float> &q)
+{
+    return static_cast<float>(static_cast<uint32_t>(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3) >> 40)) * 0x1p-24f;
+}
 
 // One global step of SSA_single_CRU (SSA.tpp:286-310); lane j holds subunit j.
-template <int PRNG, bool DIGEST>
-__device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, const SsaParams &P,
-                                 const StepConsts &K, const LccTable &T, double dt, Digest &dig, uint32_t step_lo,
-                                 int j)
+// R is the local arithmetic type (double: the reference path; float: the
+// FP32-local variant); KT the step constants of that type.
+template <int PRNG, bool DIGEST, class R, class KT>
+__device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const SsaParamsT<R> &P,
+                                 const KT &K, const LccTableT<R> &T, R dt, Digest &dig, uint32_t step_lo, int j)
 {
     QuadOut out;
     out.iters = 0;
     out.err = 0;
     // copy_from_CRUState (SSA.tpp:10-33)
     int open = ryr_get(w, 2) + ryr_get(w, 3);
-    double Jrel = P.rRyR * static_cast<double>(open) * (q.jsr - ca);
-    double t = 0.0, Jtr = 0.0, Jxfer = 0.0;
+    R Jrel = P.rRyR * static_cast<R>(open) * (q.jsr - ca);
+    R t = R(0), Jtr = R(0), Jxfer = R(0);
     while (true) {
         out.iters++;
         // update_fluxes + update_rates (SSA.tpp:37-56)
-        const double old = ca;
-        const double o_next = qnext(old);
-        const double o_prev = qprev(old);
+        const R old = ca;
+        const R o_next = qnext(old);
+        const R o_prev = qprev(old);
         Jtr = P.rtr * (K.CaNSR - q.jsr);
-        const double s_me = subunit_total(P, K, T, w, old);
-        const double s0 = qgetd<0>(s_me), s1 = qgetd<1>(s_me), s2 = qgetd<2>(s_me), s3 = qgetd<3>(s_me);
-        const double total = s0 + s1 + s2 + s3;
-        const double jump = -gw::log(qdraw<PRNG>(q)) / total;
+        const R s_me = subunit_total(P, K, T, w, old);
+        const R s0 = qget<0>(s_me), s1 = qget<1>(s_me), s2 = qget<2>(s_me), s3 = qget<3>(s_me);
+        const R total = s0 + s1 + s2 + s3;
+        const R jump = ldiv(lneglog(qdraw<PRNG>(q)), total);
         const bool last = !(t + jump < dt);
-        const double h = last ? dt - t : jump;
+        const R h = last ? dt - t : jump;
         const uint64_t w_pre = w;
         // update_CaSS (SSA.tpp:87-99) for this lane's subunit
-        const double Jiss = P.riss * (o_next + o_prev - 2 * old);
+        const R Jiss = P.riss * (o_next + o_prev - 2 * old);
         Jxfer = P.rxfer * (old - K.Cai);
-        const double betaSS = 1.0 / (1 + (P.BSR_const / sq(P.KBSR + old)) + (P.BSL_const / sq(P.KBSL + old)));
-        const double dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
-        const double tmp = old + dCaSS;
-        const int split = (old > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((old > 0.03685 && tmp <= 0.03685) ? 2 : 0);
+        const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old)) + ldiv(P.BSL_const, sq(P.KBSL + old)));
+        const R dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
+        const R tmp = old + dCaSS;
+        const int split = (old > R(1.15e-4) && tmp <= R(1.15e-4)) ? 1 : ((old > R(0.03685) && tmp <= R(0.03685)) ? 2 : 0);
         const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
         if (any_split) {
             // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
             // lane draws the same Bernoulli uniforms and the owner applies them
-            const uint64_t ob = bits_of(old);
             for (int jj = 0; jj < 4; ++jj) {
                 const int sj = qseli(split, jj);
                 if (sj) {
                     const int a_ = (sj == 1) ? 4 : 2;
                     const uint64_t wq = qselu(w, jj);
-                    const double oq = double_of(qselu(ob, jj));
-                    const double pr = (sj == 1) ? P.k65 / (P.k65 + P.k56 * sq(oq)) : P.k43 / (P.k43 + P.k34 * sq(oq));
+                    const R oq = qsel(old, jj);
+                    const R pr = (sj == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(oq)) : ldiv(P.k43, P.k43 + P.k34 * sq(oq));
                     const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
                     int X = 0;
                     for (int i = 0; i < n; ++i)
@@ -306,8 +348,8 @@ __device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, c
         ca = tmp;
         // update_CaJSR (SSA.hpp:67-71), replicated
         {
-            const double Jsum = qsum4(Jrel);
-            const double betaJSR = 1.0 / (1 + P.CSQN_const / sq(P.KCSQN + q.jsr));
+            const R Jsum = qsum4(Jrel);
+            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + q.jsr)));
             q.jsr += (h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
         }
         if (last) break;
@@ -319,8 +361,8 @@ __device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, c
         // sample_weights over the subunits (common.hpp:49-59)
         int js;
         {
-            const double u = qdraw<PRNG>(q) * total;
-            double cum = s0;
+            const R u = qdraw<PRNG>(q) * total;
+            R cum = s0;
             js = 0;
             if (cum < u) { js = 1; cum += s1;
                 if (cum < u) { js = 2; cum += s2;
@@ -328,49 +370,49 @@ __device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, c
         }
         // sample_new_state (SSA.tpp:262-283): every lane classifies its own
         // subunit; the chosen lane's class is broadcast to the quad
-        const double u3 = qdraw<PRNG>(q);
+        const R u3 = qdraw<PRNG>(q);
         const int L = sub_lcc(w);
         const int y = sub_y(w);
         const int cl = sub_cl(w);
-        const double sum_LCC = T.P[L] + T.Q[L] * old;
-        const double inact = y ? K.inact1 : K.inact0;
-        const double clr = cl ? P.kbClCh : P.kfClCh * old;
-        const double sum_RyR = s_me - (sum_LCC + inact + clr);
-        const double u = u3 * s_me;
+        const R sum_LCC = T.P[L] + T.Q[L] * old;
+        const R inact = y ? K.inact1 : K.inact0;
+        const R clr = cl ? P.kbClCh : P.kfClCh * old;
+        const R sum_RyR = s_me - (sum_LCC + inact + clr);
+        const R u = u3 * s_me;
         const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
         const int cls_q = qseli(cls, js);
-        const double u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG>(q) : 0.0;
+        const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG>(q) : R(0);
         int detail = 0;
         if (j == js) {
             if (cls == 0) {
                 // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
-                const double r0 = T.R0[L];
-                const double r1 = (L == 1) ? T.Q[L] * old : T.R1[L];
-                const double uu = u4 * sum_LCC;
-                double cum = r0;
+                const R r0 = T.R0[L];
+                const R r1 = (L == 1) ? T.Q[L] * old : T.R1[L];
+                const R uu = u4 * sum_LCC;
+                R cum = r0;
                 int tr = 0;
                 if (cum < uu) { tr = 1; cum += r1;
                     if (cum < uu) { tr = 2; } }
                 const int Ln = lcc_next(L, tr);
                 if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
-                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca) : 0.0;
+                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca) : R(0);
                 else if (L == 6 || L == 12)
-                    jl = 0.0;
+                    jl = R(0);
                 w = sub_set_lcc(w, Ln);
                 detail = Ln;
             } else if (cls == 1) {
                 const int yn = 1 - y;
                 w = sub_flip_y(w);
-                if (yn == 0) jl = 0.0;
+                if (yn == 0) jl = R(0);
                 else if (L == 6 || L == 12) jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca);
                 detail = yn;
             } else if (cls == 2) {
                 // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
-                const double uu = u4 * sum_RyR;
+                const R uu = u4 * sum_RyR;
                 int tr = 0;
-                double cum = 0.0;
+                R cum = R(0);
                 bool done = false;
-                ryr_rates(P, w_pre, old, [&](int k, double r) {
+                ryr_rates(P, w_pre, old, [&](int k, R r) {
                     if (done) return;
                     if (k == 0) { cum = r; }
                     else { if (cum < uu) { tr = k; cum += r; } else done = true; }
@@ -382,7 +424,7 @@ __device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, c
                 w = ryr_add(ryr_add(w, from, -1), to, +1);
                 if (tr == 1 || tr == 10) open += 1;
                 else if (tr == 4 || tr == 7) open -= 1;
-                Jrel = P.rRyR * static_cast<double>(open) * (q.jsr - ca);
+                Jrel = P.rRyR * static_cast<R>(open) * (q.jsr - ca);
                 detail = tr;
             } else {
                 w = sub_flip_cl(w);
@@ -391,11 +433,12 @@ __device__ QuadOut ssa_step_quad(double &ca, double &jl, uint64_t &w, CruQ &q, c
         }
         if (DIGEST) digest_event(dig, step_lo, js, cls_q, qseli(detail, js));
     }
-    // outputs of update_CRUstate_from_temp (GW.tpp:47-62) for the reductions
-    out.Jxfer = qsum4(Jxfer);
-    out.Jtr = Jtr;
-    out.JLCC = qsum4(jl);
-    out.Jrel = qsum4(Jrel);
+    // outputs of update_CRUstate_from_temp (GW.tpp:47-62) for the reductions;
+    // FP32 values enter the FP64 sums exactly
+    out.Jxfer = qsum4(static_cast<double>(Jxfer));
+    out.Jtr = static_cast<double>(Jtr);
+    out.JLCC = qsum4(static_cast<double>(jl));
+    out.Jrel = qsum4(static_cast<double>(Jrel));
     out.open = qsum4i(ryr_get(w, 2) + ryr_get(w, 3));
     out.clch = qsum4i(sub_cl(w));
     return out;

@@ -1,0 +1,101 @@
+"""The FP32-local variant against the FP64 form.
+
+CRU-local Ca, fluxes and propensities run in FP32; channel states, random
+streams, flux sums (exact, FP64) and the global ODE (FP64) are those of the
+FP64 form. The FP64 form is bit-identical to the CPU oracle (test_gpu_parity.py),
+so it stands for the reference here. The variant is not bit-exact, so its
+results are compared as ensembles: per-cell observables over independent seeds,
+means within 3 standard errors.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from .test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+PACE = {"kind": 1, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+
+
+def _run(gwcapi, ctx, precision, cells, steps):
+    b = gwcapi.Batch(ctx, cells, precision=precision)
+    b.set_mode(frozen=False, digest=False)
+    b.step(steps)
+    obs = b.obs()
+    g = b.get_globals()
+    state0 = b.get_cru_state(0)
+    b.close()
+    return obs, g, state0
+
+
+def _stats(obs):
+    apd = np.array([o["apd90_last"] for o in obs])
+    cai = np.array([o["peak_Cai"] for o in obs])
+    ical = np.array([o["peak_ICaL_neg"] for o in obs])
+    ryr = np.array([o["sum_open_RyR_frac"] / o["n_steps"] for o in obs])
+    return {"APD90": apd, "peak_Cai": cai, "peak_ICaL": ical, "mean_open_RyR": ryr}
+
+
+def _check_within_3se(a, b, name):
+    se = math.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
+    diff = a.mean() - b.mean()
+    say(f"  {name}: fp64 {b.mean():.6g} fp32 {a.mean():.6g} diff {diff:.3g} se {se:.3g} "
+        f"rel {diff / b.mean():.2e}")
+    assert abs(diff) <= 3 * se + 1e-12 * abs(b.mean()), name
+    return diff / b.mean()
+
+
+def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx):
+    """64 seeds, one paced beat (420 ms): APD90, peak Cai, peak ICaL and mean
+    RyR open fraction agree within 3 SE."""
+    seeds, n, steps = 64, 1000, 420_000
+    cells = [gwcapi.make_cell(n, 500 + s, None, PACE) for s in range(seeds)]
+    say(f"fp64 ensemble {seeds} x {n} CRUs x {steps} steps")
+    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps)
+    say("fp32-local ensemble")
+    o32, g32, s32 = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps)
+    a64, a32 = _stats(o64), _stats(o32)
+    assert np.all(a64["APD90"] > 100) and np.all(a32["APD90"] > 100)  # every cell repolarised once
+    for k in a64:
+        _check_within_3se(a32[k], a64[k], k)
+    assert all(np.isfinite(x.V) and x.V < -80 for x in g32)
+
+
+def test_fp32_local_state_invariants(gwcapi, gpu_ctx):
+    """Channel states stay exact integers with conserved RyR counts; Ca stays finite and positive."""
+    n = 4000
+    say("fp32-local invariants")
+    _, g, s = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, [gwcapi.make_cell(n, 9, None, PACE)], 30_000)
+    R = s["RyR"]
+    assert np.all(R.sum(axis=2) == 5)
+    assert np.all(R[..., 2] + R[..., 3] >= 0) and np.all(R[..., 4] + R[..., 5] >= 0)
+    assert np.all((s["LCC"] >= 1) & (s["LCC"] <= 12))
+    open_lcc = ((s["LCC"] == 6) | (s["LCC"] == 12)) & (s["LCC_inactivation"] == 1)
+    assert np.all(s["JLCC"][~open_lcc] == 0.0)
+    assert np.all(np.isfinite(s["CaSS"])) and np.all(s["CaSS"] > 0) and np.all(s["CaJSR"] > 0)
+    assert g[0].V > -40.0
+
+
+def test_fp32_local_clamp_family_errors(gwcapi, gpu_ctx):
+    """Voltage-clamp steps from -80 mV to -40..+40 mV: peak ICaL and the
+    integrated RyR release flux of the FP32-local form agree with FP64 within
+    3 SE at every step potential."""
+    seeds, n, hold_ms, step_ms = 8, 2000, 5.0, 50.0
+    volts = [-40.0, -30.0, -20.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0]
+    steps = int(round((hold_ms + step_ms) * 1000))
+    cells = []
+    for v in volts:
+        proto = {"kind": gwcapi.PROTO_CLAMP, "hold": -80.0, "step": v, "t_on": hold_ms - 5e-4,
+                 "t_off": hold_ms + step_ms}
+        cells += [gwcapi.make_cell(n, 900 + s, None, proto) for s in range(seeds)]
+    say(f"clamp family {len(volts)} x {seeds} seeds x {n} CRUs")
+    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps)
+    o32, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps)
+    for i, v in enumerate(volts):
+        sl = slice(i * seeds, (i + 1) * seeds)
+        for key in ("peak_ICaL_neg", "int_Jrel"):
+            a = np.array([o[key] for o in o32[sl]])
+            b = np.array([o[key] for o in o64[sl]])
+            _check_within_3se(a, b, f"{key} at {v:+.0f} mV")
