@@ -47,7 +47,7 @@ PKG = os.path.join(ROOT, "gw-model_amd")
 # dt - t 1, CaSS 4 x 15, CaJSR 13.
 F_ITER = 331
 STATE_BYTES = 272       # bytes read + written per CRU-step by the FP64 SSA kernel: 136 in, 136 out
-STATE_BYTES_F32 = 200   # the same for the FP32-local kernel: 100 in, 100 out
+STATE_BYTES_F32 = 240   # the same for the FP32-local kernel: 120 in, 120 out (JLCC in FP32)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0

@@ -142,7 +142,7 @@ def test_state_round_trip_and_restart(gwcapi, gpu_ctx):
     assert_globals_equal(a.get_globals()[0], b.get_globals()[0])
     assert_states_equal(a.get_cru_state(0), b.get_cru_state(0))
     s = a.get_cru_state(0)
-    a.set_cru_state(0, s)
+    a.set_cru_state(0, **s)
     assert_states_equal(a.get_cru_state(0), s, STATE_KEYS)
     a.close()
     b.close()
