@@ -191,6 +191,44 @@ __device__ void reduce_slots(Partial *red, const Partial &mine, int slot, bool w
     }
 }
 
+// Exact per-block sums of the SSA outputs. Each output's 128-bit fixed-point
+// value is split into three 43-bit limbs (the top one signed) that are added
+// into LDS with 64-bit atomics; 64 CRUs per block cannot overflow a limb sum,
+// and the limbs are recombined with carries into the same 128-bit total.
+// The integer outputs (iterations, open RyRs, open ClCh, error flags) are
+// added the same way, lane j of each quad adding output j, so that no atomic
+// address is shared by the whole wave (which would make the compiler scan
+// the active lanes one by one).
+struct BlockAcc {
+    unsigned long long limb[4][3];   // Jxfer, Jtr, JLCC, Jrel
+    unsigned long long ints[4];      // iterations, open RyRs, open ClCh, error flags
+};
+
+__device__ __forceinline__ void acc_zero(BlockAcc &A)
+{
+    const int t = threadIdx.x;
+    if (t < 16) (&A.limb[0][0])[t] = 0;
+}
+
+__device__ __forceinline__ void acc_add_fx(BlockAcc &A, int q, const Fx &f)
+{
+    const uint64_t m43 = (1ULL << 43) - 1;
+    atomicAdd(&A.limb[q][0], static_cast<unsigned long long>(f.lo & m43));
+    atomicAdd(&A.limb[q][1], static_cast<unsigned long long>((f.lo >> 43) | ((f.hi & ((1ULL << 22) - 1)) << 21)));
+    atomicAdd(&A.limb[q][2], static_cast<unsigned long long>(static_cast<int64_t>(f.hi) >> 22));
+}
+
+__device__ __forceinline__ Fx acc_fx(const BlockAcc &A, int q)
+{
+    const uint64_t S0 = A.limb[q][0], S1 = A.limb[q][1], S2 = A.limb[q][2];
+    Fx f;
+    f.lo = S0;
+    f.hi = 0;
+    fx_add_raw(f, S1 << 43, S1 >> 21);
+    fx_add_raw(f, 0, S2 << 22);
+    return f;
+}
+
 template <int NT>
 __device__ void block_reduce(Partial *red, Partial mine)
 {
@@ -310,42 +348,27 @@ __device__ __forceinline__ void load_lcc_table(const KT &K, TT &T)
     for (int w = threadIdx.x; w < static_cast<int>(sizeof(TT) / 4); w += blockDim.x) dst[w] = src[w];
 }
 
-__device__ __forceinline__ void partial_from(Partial &m, const QuadOut &o, uint64_t iters)
-{
-    m.err |= fx_add(m.jlcc, o.JLCC);
-    m.err |= fx_add(m.jxfer, o.Jxfer);
-    m.err |= fx_add(m.jtr, o.Jtr);
-    fx_add(m.jrel, o.Jrel);
-    m.clch = o.clch;
-    m.open = o.open;
-    m.iters = iters;
-    m.err |= o.err ? 4 : 0;
-}
 
 // SSA(dt) of `nsteps` steps (GW_model::SSA, src/includes/GW.tpp:214-234) for
 // the 64 CRUs of SSA block b, with the constants of step step0 (nsteps > 1
 // only in frozen-globals replay, where they do not change). The block's flux
 // partial of the last step goes to partials[step0 & 1][b]; only the quad
 // leaders contribute. R is the CRU-local arithmetic (double, or float for the
-// FP32-local variant).
+// FP32-local variant); CaSS and CaJSR are FP64 in both.
 template <class R> struct LocalState;
 template <> struct LocalState<double> {
-    static __device__ __forceinline__ double *cass(const KArgs &a) { return a.CaSS; }
     static __device__ __forceinline__ double *jlcc(const KArgs &a) { return a.JLCC; }
-    static __device__ __forceinline__ double *cajsr(const KArgs &a) { return a.CaJSR; }
     static __device__ __forceinline__ const SsaParams &params(const DevCell &C) { return C.sp; }
     static __device__ __forceinline__ const StepConsts &consts(const CellDyn &D, int ks) { return D.k[ks]; }
 };
 template <> struct LocalState<float> {
-    static __device__ __forceinline__ float *cass(const KArgs &a) { return a.CaSSf; }
     static __device__ __forceinline__ float *jlcc(const KArgs &a) { return a.JLCCf; }
-    static __device__ __forceinline__ float *cajsr(const KArgs &a) { return a.CaJSRf; }
     static __device__ __forceinline__ const SsaParamsF &params(const DevCell &C) { return C.spf; }
     static __device__ __forceinline__ const StepConstsF &consts(const CellDyn &D, int ks) { return D.kf[ks]; }
 };
 
 template <int PRNG, bool DIGEST, class R>
-__device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTableT<R> &T, Partial *red)
+__device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTableT<R> &T, BlockAcc &acc)
 {
     using LS = LocalState<R>;
     const int tid = threadIdx.x;
@@ -357,21 +380,18 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     const auto &sp = LS::params(C);
     const auto &kk = LS::consts(D, ks);
     load_lcc_table(kk, T);
+    acc_zero(acc);
     __syncthreads();
     const int64_t i = (b - C.blk0) * kCruPerBlock + (tid >> 2);
     const bool active = (i < C.n_cru) && (D.status == 0);
-    Partial mine;
-    partial_zero(mine);
     if (active) {
-        R *const CaSS = LS::cass(a);
         R *const JLCC = LS::jlcc(a);
-        R *const CaJSR = LS::cajsr(a);
         const int64_t slot = C.slot0 + i;
-        R ca = CaSS[slot * 4 + j];
+        double ca = a.CaSS[slot * 4 + j];
         R jl = JLCC[slot * 4 + j];
         uint64_t w = a.sub[slot * 4 + j];
-        CruQT<R> q;
-        q.jsr = CaJSR[slot];
+        CruQ q;
+        q.jsr = a.CaJSR[slot];
         q.r0 = a.rng[slot * 4];
         q.r1 = a.rng[slot * 4 + 1];
         q.r2 = a.rng[slot * 4 + 2];
@@ -389,11 +409,11 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
             iters += o.iters;
             if (o.err) break;
         }
-        CaSS[slot * 4 + j] = ca;
+        a.CaSS[slot * 4 + j] = ca;
         JLCC[slot * 4 + j] = jl;
         a.sub[slot * 4 + j] = w;
         if (j == 0) {
-            CaJSR[slot] = q.jsr;
+            a.CaJSR[slot] = q.jsr;
             a.rng[slot * 4] = q.r0;
             a.rng[slot * 4 + 1] = q.r1;
             a.rng[slot * 4 + 2] = q.r2;
@@ -402,11 +422,32 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
                 a.dig[slot * 2] = dg.count;
                 a.dig[slot * 2 + 1] = dg.hash;
             }
-            partial_from(mine, o, iters);
         }
+        // lane j adds output j of its CRU (update_CRUstate_from_temp, GW.tpp:47-62)
+        const double v = j == 0 ? o.Jxfer : j == 1 ? o.Jtr : j == 2 ? o.JLCC : o.Jrel;
+        Fx f;
+        fx_zero(f);
+        const int fl = fx_add(f, v);
+        acc_add_fx(acc, j, f);
+        // error flags of the three summed fluxes (the range of Jrel is not checked)
+        const int e = (qgeti<0>(fl) | qgeti<1>(fl) | qgeti<2>(fl)) | (o.err ? 4 : 0);
+        const int64_t iv = j == 0 ? static_cast<int64_t>(iters) : j == 1 ? o.open : j == 2 ? o.clch : e;
+        atomicAdd(&acc.ints[j], static_cast<unsigned long long>(iv));
     }
-    reduce_slots<kCruPerBlock>(red, mine, tid >> 2, j == 0);
-    if (tid == 0) a.partials[ks * a.nblocks + b] = red[0];
+    __syncthreads();
+    if (tid == 0) {
+        Partial p;
+        p.jxfer = acc_fx(acc, 0);
+        p.jtr = acc_fx(acc, 1);
+        p.jlcc = acc_fx(acc, 2);
+        p.jrel = acc_fx(acc, 3);
+        p.iters = acc.ints[0];
+        p.open = static_cast<int64_t>(acc.ints[1]);
+        p.clch = static_cast<int64_t>(acc.ints[2]);
+        p.err = acc.ints[3] ? 1 : 0;
+        p._pad = 0;
+        a.partials[ks * a.nblocks + b] = p;
+    }
 }
 
 // Close step n = D.step of one cell: the exact sum of the cell's block
@@ -476,12 +517,13 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_step(KArgs a, int6
 {
     __shared__ LccTableT<R> T;
     __shared__ Partial red[kCruPerBlock];
+    __shared__ BlockAcc acc;
     const int bid = blockIdx.x;
     if (bid < nclose) {
         close_cell<kCruPerBlock>(a, bid, red);
         return;
     }
-    ssa_block<PRNG, DIGEST, R>(a, bid - nclose, ssa_step, 1, T, red);
+    ssa_block<PRNG, DIGEST, R>(a, bid - nclose, ssa_step, 1, T, acc);
 }
 
 // Frozen-globals replay: nsteps SSA steps in one launch.
@@ -489,8 +531,8 @@ template <int PRNG, bool DIGEST, class R>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
 {
     __shared__ LccTableT<R> T;
-    __shared__ Partial red[kCruPerBlock];
-    ssa_block<PRNG, DIGEST, R>(a, blockIdx.x, step0, nsteps, T, red);
+    __shared__ BlockAcc acc;
+    ssa_block<PRNG, DIGEST, R>(a, blockIdx.x, step0, nsteps, T, acc);
 }
 
 // Close a frozen replay of nsteps: sums of the last step, clock advanced as
@@ -948,7 +990,7 @@ int gw_batch_destroy(gw_batch *B)
     live.erase(std::remove(live.begin(), live.end(), B), live.end());
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
-    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.CaSSf, B->a.CaJSRf, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
+    void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
                     B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1027,10 +1069,9 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     KArgs &a = B->a;
     int rc = GW_OK;
     const bool f32 = precision == GW_PREC_FP32_LOCAL;
-    if ((!f32 && ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
-                  (rc = dmalloc(&a.JLCC, 4 * N, err, errlen)))) ||
-        (f32 && ((rc = dmalloc(&a.CaSSf, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSRf, N, err, errlen)) ||
-                 (rc = dmalloc(&a.JLCCf, 4 * N, err, errlen)))) ||
+    if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
+        (!f32 && (rc = dmalloc(&a.JLCC, 4 * N, err, errlen))) ||
+        (f32 && (rc = dmalloc(&a.JLCCf, 4 * N, err, errlen))) ||
         (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
         (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->nblocks, err, errlen)) ||

@@ -54,12 +54,10 @@ struct KArgs {
     const DevCell *cells;
     CellDyn *dyn;
     const int32_t *block_cell;
-    double *CaSS;      // [ntot][4]   (FP64 batches)
+    double *CaSS;      // [ntot][4]   (both forms: Ca accumulates in FP64)
     double *CaJSR;     // [ntot]
-    double *JLCC;      // [ntot][4]
-    float *CaSSf;      // [ntot][4]   (FP32-local batches)
-    float *CaJSRf;     // [ntot]
-    float *JLCCf;      // [ntot][4]
+    double *JLCC;      // [ntot][4]   (FP64 batches)
+    float *JLCCf;      // [ntot][4]   (FP32-local batches)
     uint64_t *sub;     // [ntot][4] packed subunit words (RyR counts, LCC, y, ClCh)
     uint64_t *rng;     // [ntot][4] engine words
     uint64_t *dig;     // [ntot][2] (count, hash) when digests are kept
@@ -79,20 +77,14 @@ struct KArgs {
 };
 
 // CRU-local state of either precision, for the kernels outside the SSA.
-__device__ __forceinline__ double ld_cass(const KArgs &a, int64_t k) { return a.CaSSf ? static_cast<double>(a.CaSSf[k]) : a.CaSS[k]; }
+__device__ __forceinline__ double ld_cass(const KArgs &a, int64_t k) { return a.CaSS[k]; }
 __device__ __forceinline__ double ld_jlcc(const KArgs &a, int64_t k) { return a.JLCCf ? static_cast<double>(a.JLCCf[k]) : a.JLCC[k]; }
-__device__ __forceinline__ double ld_cajsr(const KArgs &a, int64_t k) { return a.CaJSRf ? static_cast<double>(a.CaJSRf[k]) : a.CaJSR[k]; }
-__device__ __forceinline__ void st_cass(const KArgs &a, int64_t k, double v)
-{
-    if (a.CaSSf) a.CaSSf[k] = static_cast<float>(v); else a.CaSS[k] = v;
-}
+__device__ __forceinline__ double ld_cajsr(const KArgs &a, int64_t k) { return a.CaJSR[k]; }
+__device__ __forceinline__ void st_cass(const KArgs &a, int64_t k, double v) { a.CaSS[k] = v; }
 __device__ __forceinline__ void st_jlcc(const KArgs &a, int64_t k, double v)
 {
     if (a.JLCCf) a.JLCCf[k] = static_cast<float>(v); else a.JLCC[k] = v;
 }
-__device__ __forceinline__ void st_cajsr(const KArgs &a, int64_t k, double v)
-{
-    if (a.CaJSRf) a.CaJSRf[k] = static_cast<float>(v); else a.CaJSR[k] = v;
-}
+__device__ __forceinline__ void st_cajsr(const KArgs &a, int64_t k, double v) { a.CaJSR[k] = v; }
 
 }  // namespace gw

@@ -133,6 +133,7 @@ GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c,
 
 // The SSA constants of a step rounded to FP32, for the FP32-local SSA.
 struct StepConstsF {
+    double Cai_d, CaNSR_d;   // kept in FP64: they enter the SSA only through differences
     float Cai, CaNSR;
     float inact0, inact1;
     float JLCC_mult, JLCC_exp;
@@ -141,6 +142,8 @@ struct StepConstsF {
 
 GW_HD void step_consts_f32(StepConstsF &f, const StepConsts &k)
 {
+    f.Cai_d = k.Cai;
+    f.CaNSR_d = k.CaNSR;
     f.Cai = static_cast<float>(k.Cai);
     f.CaNSR = static_cast<float>(k.CaNSR);
     f.inact0 = static_cast<float>(k.inact0);

@@ -17,9 +17,13 @@
 //     threshold re-split), so every comparison sees identical values;
 //   - all arithmetic keeps the reference's operation order (in FP64,
 //     bit-exact with the CPU oracle when compiled with -ffp-contract=off);
-//   - the FP32-local variant runs the same code on floats: CRU-local Ca,
-//     fluxes and propensities in FP32 with hardware reciprocal and log2, the
-//     integer channel states and the random streams unchanged.
+//   - the FP32-local variant runs the same code with R = float: propensities,
+//     fluxes, buffering factors and waiting times in FP32 with the hardware
+//     reciprocal and log2; the integer channel states and the random streams
+//     are unchanged. CaSS and CaJSR keep FP64 accumulators (each FP32
+//     increment is added to an FP64 running value) because their per-step
+//     increments near rest are below one FP32 ulp (CaJSR: about 0.6 ulp), and
+//     the concentration differences that drive fluxes are formed in FP64.
 #pragma once
 
 #include "gw_model.h"
@@ -259,13 +263,17 @@ template <class R>
 __device__ __forceinline__ R qsum4(R v) { return qget<0>(v) + qget<1>(v) + qget<2>(v) + qget<3>(v); }
 __device__ __forceinline__ int qsum4i(int v) { return qgeti<0>(v) + qgeti<1>(v) + qgeti<2>(v) + qgeti<3>(v); }
 
-// Replicated CRU-level state of a quad.
-template <class R>
-struct CruQT {
-    R jsr;
+// Replicated CRU-level state of a quad (CaJSR accumulates in FP64 in both forms).
+struct CruQ {
+    double jsr;
     uint64_t r0, r1, r2, r3;
 };
-using CruQ = CruQT<double>;
+
+// Cai and CaNSR of a step at the accumulator precision.
+GW_HD double kcai(const StepConsts &k) { return k.Cai; }
+GW_HD double kcansr(const StepConsts &k) { return k.CaNSR; }
+GW_HD double kcai(const StepConstsF &k) { return k.Cai_d; }
+GW_HD double kcansr(const StepConstsF &k) { return k.CaNSR_d; }
 
 // Per-CRU outputs of one step for the reductions (identical in the 4 lanes).
 struct QuadOut {
@@ -280,49 +288,55 @@ struct QuadOut {
 };
 
 // Uniform draws: FP64 uses the reference's 64-bit map; FP32 the top 24 bits.
-template <int PRNG>
-__device__ __forceinline__ double qdraw(CruQT<double> &q) { return u01(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3)); }
-template <int PRNG>
-__device__ __forceinline__ float qdraw(CruQT<float> &q)
+template <int PRNG, class R>
+__device__ __forceinline__ R qdraw(CruQ &q)
 {
-    return static_cast<float>(static_cast<uint32_t>(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3) >> 40)) * 0x1p-24f;
+    if constexpr (sizeof(R) == 8) {
+        return u01(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3));
+    } else {
+        return static_cast<float>(static_cast<uint32_t>(rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3) >> 40)) * 0x1p-24f;
+    }
 }
 
 // One global step of SSA_single_CRU (SSA.tpp:286-310); lane j holds subunit j.
 // R is the local arithmetic type (double: the reference path; float: the
-// FP32-local variant); KT the step constants of that type.
+// FP32-local variant); KT the step constants of that type. CaSS (`ca`) and
+// CaJSR accumulate in FP64 in both; with R = double every expression below is
+// the reference's, in its order.
 template <int PRNG, bool DIGEST, class R, class KT>
-__device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const SsaParamsT<R> &P,
+__device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const SsaParamsT<R> &P,
                                  const KT &K, const LccTableT<R> &T, R dt, Digest &dig, uint32_t step_lo, int j)
 {
     QuadOut out;
     out.iters = 0;
     out.err = 0;
+    const double Cai = kcai(K), CaNSR = kcansr(K);
     // copy_from_CRUState (SSA.tpp:10-33)
     int open = ryr_get(w, 2) + ryr_get(w, 3);
-    R Jrel = P.rRyR * static_cast<R>(open) * (q.jsr - ca);
+    R Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
     R t = R(0), Jtr = R(0), Jxfer = R(0);
     while (true) {
         out.iters++;
         // update_fluxes + update_rates (SSA.tpp:37-56)
-        const R old = ca;
-        const R o_next = qnext(old);
-        const R o_prev = qprev(old);
-        Jtr = P.rtr * (K.CaNSR - q.jsr);
+        const double oldA = ca;
+        const R old = static_cast<R>(oldA);
+        const double o_next = qnext(oldA);
+        const double o_prev = qprev(oldA);
+        Jtr = P.rtr * static_cast<R>(CaNSR - q.jsr);
         const R s_me = subunit_total(P, K, T, w, old);
         const R s0 = qget<0>(s_me), s1 = qget<1>(s_me), s2 = qget<2>(s_me), s3 = qget<3>(s_me);
         const R total = s0 + s1 + s2 + s3;
-        const R jump = ldiv(lneglog(qdraw<PRNG>(q)), total);
+        const R jump = ldiv(lneglog(qdraw<PRNG, R>(q)), total);
         const bool last = !(t + jump < dt);
         const R h = last ? dt - t : jump;
         const uint64_t w_pre = w;
         // update_CaSS (SSA.tpp:87-99) for this lane's subunit
-        const R Jiss = P.riss * (o_next + o_prev - 2 * old);
-        Jxfer = P.rxfer * (old - K.Cai);
+        const R Jiss = P.riss * static_cast<R>(o_next + o_prev - 2 * oldA);
+        Jxfer = P.rxfer * static_cast<R>(oldA - Cai);
         const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old)) + ldiv(P.BSL_const, sq(P.KBSL + old)));
         const R dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
-        const R tmp = old + dCaSS;
-        const int split = (old > R(1.15e-4) && tmp <= R(1.15e-4)) ? 1 : ((old > R(0.03685) && tmp <= R(0.03685)) ? 2 : 0);
+        const double tmp = oldA + static_cast<double>(dCaSS);
+        const int split = (oldA > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((oldA > 0.03685 && tmp <= 0.03685) ? 2 : 0);
         const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
         if (any_split) {
             // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
@@ -337,7 +351,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
                     const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
                     int X = 0;
                     for (int i = 0; i < n; ++i)
-                        if (qdraw<PRNG>(q) < pr) X++;
+                        if (qdraw<PRNG, R>(q) < pr) X++;
                     const int na = (n > 0) ? X : 0;
                     const int nb = (n > 0) ? n - X : 0;
                     if (j == jj) w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
@@ -349,8 +363,8 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
         // update_CaJSR (SSA.hpp:67-71), replicated
         {
             const R Jsum = qsum4(Jrel);
-            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + q.jsr)));
-            q.jsr += (h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
+            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(q.jsr))));
+            q.jsr += static_cast<double>(h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
         }
         if (last) break;
         if (out.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
@@ -361,7 +375,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
         // sample_weights over the subunits (common.hpp:49-59)
         int js;
         {
-            const R u = qdraw<PRNG>(q) * total;
+            const R u = qdraw<PRNG, R>(q) * total;
             R cum = s0;
             js = 0;
             if (cum < u) { js = 1; cum += s1;
@@ -370,7 +384,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
         }
         // sample_new_state (SSA.tpp:262-283): every lane classifies its own
         // subunit; the chosen lane's class is broadcast to the quad
-        const R u3 = qdraw<PRNG>(q);
+        const R u3 = qdraw<PRNG, R>(q);
         const int L = sub_lcc(w);
         const int y = sub_y(w);
         const int cl = sub_cl(w);
@@ -381,9 +395,10 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
         const R u = u3 * s_me;
         const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
         const int cls_q = qseli(cls, js);
-        const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG>(q) : R(0);
+        const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG, R>(q) : R(0);
         int detail = 0;
         if (j == js) {
+            const R caR = static_cast<R>(ca);
             if (cls == 0) {
                 // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
                 const R r0 = T.R0[L];
@@ -395,7 +410,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
                     if (cum < uu) { tr = 2; } }
                 const int Ln = lcc_next(L, tr);
                 if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
-                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca) : R(0);
+                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR) : R(0);
                 else if (L == 6 || L == 12)
                     jl = R(0);
                 w = sub_set_lcc(w, Ln);
@@ -404,7 +419,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
                 const int yn = 1 - y;
                 w = sub_flip_y(w);
                 if (yn == 0) jl = R(0);
-                else if (L == 6 || L == 12) jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * ca);
+                else if (L == 6 || L == 12) jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR);
                 detail = yn;
             } else if (cls == 2) {
                 // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
@@ -424,7 +439,7 @@ __device__ QuadOut ssa_step_quad(R &ca, R &jl, uint64_t &w, CruQT<R> &q, const S
                 w = ryr_add(ryr_add(w, from, -1), to, +1);
                 if (tr == 1 || tr == 10) open += 1;
                 else if (tr == 4 || tr == 7) open -= 1;
-                Jrel = P.rRyR * static_cast<R>(open) * (q.jsr - ca);
+                Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
                 detail = tr;
             } else {
                 w = sub_flip_cl(w);
