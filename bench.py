@@ -64,13 +64,18 @@ def pacing(capi):
     return {"kind": capi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
 
 
-def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0):
+def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0, autotune=200):
     ctx = capi.Context(0)
     b = capi.Batch(ctx, [capi.make_cell(crus, seed0 + c, None, pacing(capi)) for c in range(cells)],
                    precision=precision)
     b.set_mode(frozen=False, digest=False)
     if warmup > 0:
         b.step(warmup)
+    tune = None
+    if autotune > 0:  # trials start from the warmed-up state, which is restored afterwards
+        blk, trials = b.autotune(autotune)
+        tune = {"block": blk, "crus_per_block": blk // 4, "trial_steps": autotune,
+                "trial_ms_per_step": {str(k): v for k, v in trials.items()}}
     i0 = b.iterations()
     ms, ssa_ms, launches = b.time_steps(steps, per_kernel=True)
     iters = b.iterations() - i0
@@ -78,7 +83,7 @@ def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0):
     obs = b.obs()
     b.close()
     ctx.close()
-    return ms, ssa_ms, launches, iters, g, obs
+    return ms, ssa_ms, launches, iters, g, obs, tune
 
 
 def measured_traffic(total_crus, form):
@@ -142,8 +147,8 @@ def copy_calibration(nbytes, reps=5):
 def measure(capi, precision, args, total_crus):
     """One form (FP64 or FP32-local) on the bench batch: throughput and the SSA kernel's roofline."""
     f32 = precision == capi.PREC_FP32_LOCAL
-    ms, ssa_ms, launches, iters, g, obs = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup,
-                                                    precision=precision)
+    ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, args.cells, args.crus, args.steps, args.warmup,
+                                                          precision=precision, autotune=args.autotune)
     sim_ms = args.steps * 1e-3
     sec = ms / 1e3
     per_launch_s = ssa_ms / 1e3 / launches
@@ -161,6 +166,7 @@ def measure(capi, precision, args, total_crus):
         "cell_beats_per_s": args.cells * (sim_ms / 1000.0) / sec,
         "iterations_per_cru_step": iters / (launches * total_crus),
         "mean_V_end": float(np.mean([x.V for x in g])),
+        "launch": tune,
         "roofline": {
             "bound": "valu" if valu else "hbm",
             "achieved": achieved_tflops if valu else achieved_gbs,
@@ -189,6 +195,8 @@ def main(argv=None):
     ap.add_argument("--cpu-crus", type=int, default=12500)
     ap.add_argument("--cpu-steps", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--autotune", type=int, default=200, metavar="STEPS",
+                    help="autotuner trial steps per SSA block size before the timed region (0: default shape)")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -228,11 +236,12 @@ def main(argv=None):
                    "cells": args.cells, "crus_per_cell": args.crus, "dt_ms": 1e-3, "precision": chosen[0]},
         "cell_beats_per_s": head["cell_beats_per_s"],
         "iterations_per_cru_step": head["iterations_per_cru_step"],
+        "launch": head["launch"],
         "roofline": head["roofline"],
     }
     for k in chosen[1:]:
         line[k] = {x: res[k][x] for x in ("value", "ms_per_step", "cell_beats_per_s", "iterations_per_cru_step",
-                                          "roofline")}
+                                          "launch", "roofline")}
     if len(chosen) == 2:
         line["speedup_fp32_local_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     value = head["value"]

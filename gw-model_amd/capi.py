@@ -138,6 +138,8 @@ def lib() -> C.CDLL:
         "gw_batch_get_iterations": (C.c_int, [P, C.POINTER(u64)]),
         "gw_batch_set_launch": (C.c_int, [P, i32, i32] + E),
         "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
+        "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
+        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(f64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
     }
     for name, (res, args) in sig.items():
@@ -306,6 +308,19 @@ class Batch:
 
     def set_launch(self, crus_per_thread: int = 0, block: int = 0):
         _call(self.L.gw_batch_set_launch, self.h, crus_per_thread, block)
+
+    def launch(self) -> int:
+        """Threads per block of the current SSA launch shape."""
+        cpt, blk = i32(), i32()
+        self.L.gw_batch_get_launch(self.h, C.byref(cpt), C.byref(blk))
+        return blk.value
+
+    def autotune(self, trial_steps: int = 200) -> tuple[int, dict]:
+        """gw_batch_autotune: pick the fastest SSA block size on this batch (its state is left unchanged)."""
+        blk = i32()
+        ms = (f64 * 4)()
+        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(blk), ms)
+        return blk.value, {64 << k: (ms[k] if ms[k] >= 0 else None) for k in range(4)}
 
     def time_steps(self, nsteps: int, per_kernel: bool = False):
         """(total ms, SSA-kernel ms or None, SSA launches) of nsteps Euler steps."""

@@ -253,7 +253,7 @@ __global__ void k_seed(KArgs a)
     const int cell = a.block_cell[b];
     const DevCell &C = a.cells[cell];
     const int j = threadIdx.x & 3;
-    const int64_t i = (b - C.blk0) * kCruPerBlock + (threadIdx.x >> 2);
+    const int64_t i = (b - C.blk0[kShapeDefault]) * kCruPerBlock + (threadIdx.x >> 2);
     const int64_t slot = C.slot0 + i;
     if (j == 0) {
         uint64_t s[4];
@@ -280,7 +280,7 @@ __global__ void k_sample_rest(KArgs a)
     const int cell = a.block_cell[b];
     const DevCell &C = a.cells[cell];
     if ((threadIdx.x & 3) != 0) return;
-    const int64_t i = (b - C.blk0) * kCruPerBlock + (threadIdx.x >> 2);
+    const int64_t i = (b - C.blk0[kShapeDefault]) * kCruPerBlock + (threadIdx.x >> 2);
     if (i >= C.n_cru) return;
     const int64_t slot = C.slot0 + i;
     double ca[4], jsr;
@@ -367,9 +367,10 @@ template <> struct LocalState<float> {
     static __device__ __forceinline__ const StepConstsF &consts(const CellDyn &D, int ks) { return D.kf[ks]; }
 };
 
-template <int PRNG, bool DIGEST, class R>
+template <int PRNG, bool DIGEST, class R, int BLOCK>
 __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, int64_t nsteps, LccTableT<R> &T, BlockAcc &acc)
 {
+    constexpr int kShape = shape_of_block(BLOCK);
     using LS = LocalState<R>;
     const int tid = threadIdx.x;
     const int j = tid & 3;
@@ -382,7 +383,7 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     load_lcc_table(kk, T);
     acc_zero(acc);
     __syncthreads();
-    const int64_t i = (b - C.blk0) * kCruPerBlock + (tid >> 2);
+    const int64_t i = (b - C.blk0[kShape]) * (BLOCK / 4) + (tid >> 2);
     const bool active = (i < C.n_cru) && (D.status == 0);
     if (active) {
         R *const JLCC = LS::jlcc(a);
@@ -465,7 +466,7 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *re
     Partial mine;
     partial_zero(mine);
     if (tid < NS)
-        for (int q = tid; q < C.nblk; q += NS) partial_add(mine, a.partials[ks * a.nblocks + C.blk0 + q]);
+        for (int q = tid; q < C.nblk[a.shape]; q += NS) partial_add(mine, a.partials[ks * a.nblocks + C.blk0[a.shape] + q]);
     reduce_slots<NS>(red, mine, tid, tid < NS);
     if (threadIdx.x != 0) return;
     const Partial &tot = red[0];
@@ -512,18 +513,18 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *re
 // spills to scratch, which it can afford).
 template <class R> constexpr int kWavesPerSimd = sizeof(R) == 4 ? 6 : 4;
 
-template <int PRNG, bool DIGEST, class R>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
+template <int PRNG, bool DIGEST, class R, int BLOCK>
+__global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
 {
     __shared__ LccTableT<R> T;
-    __shared__ Partial red[kCruPerBlock];
+    __shared__ Partial red[kWave];
     __shared__ BlockAcc acc;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kCruPerBlock>(a, bid, red);
+        close_cell<kWave>(a, bid, red);
         return;
     }
-    ssa_block<PRNG, DIGEST, R>(a, bid - nclose, ssa_step, 1, T, acc);
+    ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, ssa_step, 1, T, acc);
 }
 
 // Frozen-globals replay: nsteps SSA steps in one launch.
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_ssa(KArgs a, int64
 {
     __shared__ LccTableT<R> T;
     __shared__ BlockAcc acc;
-    ssa_block<PRNG, DIGEST, R>(a, blockIdx.x, step0, nsteps, T, acc);
+    ssa_block<PRNG, DIGEST, R, kBlock>(a, blockIdx.x, step0, nsteps, T, acc);
 }
 
 // Close a frozen replay of nsteps: sums of the last step, clock advanced as
@@ -544,7 +545,7 @@ __global__ void k_frozen_close(KArgs a, int64_t step0, int64_t nsteps)
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
     Partial tot;
-    sum_block_partials(a.partials + (step0 & 1) * a.nblocks, C.blk0, C.nblk, tot);
+    sum_block_partials(a.partials + (step0 & 1) * a.nblocks, C.blk0[a.shape], C.nblk[a.shape], tot);
     if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
     if (tot.err && D.status == 0) D.status = GW_E_RANGE;
     D.sums[0] = fx_to_double(tot.jlcc); D.sums[1] = fx_to_double(tot.jxfer); D.sums[2] = fx_to_double(tot.jtr);
@@ -669,7 +670,10 @@ struct gw_batch {
     int64_t steps_done = 0;
     int32_t trace_every = 0;
     int64_t trace_cap = 0;
-    int32_t crus_per_thread = 1, block = kBlock;
+    int32_t shape = kShapeDefault;
+    int64_t nblocks_shape[kNumShapes] = {0, 0, 0, 0};
+    int64_t bc_off[kNumShapes] = {0, 0, 0, 0};   // offset of each shape's block -> cell map
+    int64_t max_nblocks = 0;
     std::vector<DevCell> cells_h;
     std::vector<int32_t> block_cell_h;
     std::vector<gw_globals_t> g0;
@@ -712,6 +716,23 @@ int launch_cfg_check(gw_batch *B, char *err, size_t errlen)
     return GW_OK;
 }
 
+void set_shape(gw_batch *B, int s)
+{
+    B->shape = s;
+    B->nblocks = B->nblocks_shape[s];
+    B->a.shape = s;
+    B->a.nblocks = B->nblocks_shape[s];
+    B->a.block_cell = B->d_block_cell + B->bc_off[s];
+}
+
+// Shapes other than the default are built for the fast path only: FP64
+// without replay digests or FP32-local, xoshiro256**, free-running globals.
+bool shape_allowed(const gw_batch *B, int s)
+{
+    return s == kShapeDefault || (B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen &&
+                                  (B->precision == GW_PREC_FP32_LOCAL || !B->digest));
+}
+
 int prepare(gw_batch *B, char *err, size_t errlen)
 {
     if (B->prepared) return GW_OK;
@@ -751,16 +772,37 @@ void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
 // One launch of k_step: SSA(ssa_step) (none if ssa_step < 0) beside the
 // closes of the previous step (when nclose > 0). When e0/e1 are given, the
 // dispatch itself records them (its start and end, no extra queue packets).
+template <int PRNG, int BLOCK>
+void launch_k_step_shape(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
+{
+    const unsigned grid = static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
+    if (B->precision == GW_PREC_FP32_LOCAL) {
+        hipExtLaunchKernelGGL((k_step<PRNG, false, float, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+        return;
+    }
+    if constexpr (BLOCK == kBlock) {
+        if (B->digest) {
+            hipExtLaunchKernelGGL((k_step<PRNG, true, double, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+            return;
+        }
+    }
+    hipExtLaunchKernelGGL((k_step<PRNG, false, double, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+}
+
+// Shapes other than the default are built for xoshiro256** only (the engine
+// of the production path); shape_allowed() keeps the others off them.
 template <int PRNG>
 void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
 {
-    const unsigned grid = static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
-    if (B->precision == GW_PREC_FP32_LOCAL)
-        hipExtLaunchKernelGGL((k_step<PRNG, false, float>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
-    else if (B->digest)
-        hipExtLaunchKernelGGL((k_step<PRNG, true, double>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
-    else
-        hipExtLaunchKernelGGL((k_step<PRNG, false, double>), dim3(grid), dim3(kBlock), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+    if constexpr (PRNG == GW_PRNG_XOSHIRO256SS) {
+        switch (B->shape) {
+        case 0: launch_k_step_shape<PRNG, 64>(B, ssa_step, nclose, e0, e1); return;
+        case 1: launch_k_step_shape<PRNG, 128>(B, ssa_step, nclose, e0, e1); return;
+        case 3: launch_k_step_shape<PRNG, 512>(B, ssa_step, nclose, e0, e1); return;
+        default: break;
+        }
+    }
+    launch_k_step_shape<PRNG, kBlock>(B, ssa_step, nclose, e0, e1);
 }
 
 void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
@@ -1029,7 +1071,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     B->dt = dt;
     B->cells_h.resize(ncells);
     B->istim_dev.assign(ncells, nullptr);
-    int64_t slot = 0, blk = 0;
+    int64_t slot = 0;
     for (int c = 0; c < ncells; ++c) {
         const gw_cell_desc_t &d = cells[c];
         if (d.n_cru <= 0 || d.n_cru > (1 << 20)) {
@@ -1056,15 +1098,27 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         C.proto = d.protocol;
         C.seed = d.seed;
         C.n_cru = d.n_cru;
-        C.nblk = (d.n_cru + kCruPerBlock - 1) / kCruPerBlock;
         C.slot0 = slot;
-        C.blk0 = blk;
-        slot += static_cast<int64_t>(C.nblk) * kCruPerBlock;
-        blk += C.nblk;
-        for (int b = 0; b < C.nblk; ++b) B->block_cell_h.push_back(c);
+        for (int sh = 0; sh < kNumShapes; ++sh) {
+            const int per = shape_threads(sh) / 4;
+            C.nblk[sh] = (d.n_cru + per - 1) / per;
+        }
+        slot += static_cast<int64_t>((d.n_cru + kSlotAlign - 1) / kSlotAlign) * kSlotAlign;
+    }
+    for (int sh = 0; sh < kNumShapes; ++sh) {
+        B->bc_off[sh] = static_cast<int64_t>(B->block_cell_h.size());
+        int64_t blk = 0;
+        for (int c = 0; c < ncells; ++c) {
+            DevCell &C = B->cells_h[c];
+            C.blk0[sh] = blk;
+            blk += C.nblk[sh];
+            for (int b = 0; b < C.nblk[sh]; ++b) B->block_cell_h.push_back(c);
+        }
+        B->nblocks_shape[sh] = blk;
+        B->max_nblocks = std::max(B->max_nblocks, blk);
     }
     B->ntot = slot;
-    B->nblocks = blk;
+    B->nblocks = B->nblocks_shape[kShapeDefault];
     const int64_t N = B->ntot;
     KArgs &a = B->a;
     int rc = GW_OK;
@@ -1074,16 +1128,17 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         (f32 && (rc = dmalloc(&a.JLCCf, 4 * N, err, errlen))) ||
         (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
-        (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->nblocks, err, errlen)) ||
+        (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->max_nblocks, err, errlen)) ||
         (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
-        (rc = dmalloc(&B->d_block_cell, B->nblocks, err, errlen))) {
+        (rc = dmalloc(&B->d_block_cell, B->block_cell_h.size(), err, errlen))) {
         gw_batch_destroy(B);
         return rc;
     }
     a.cells = B->d_cells;
     a.dyn = B->d_dyn;
-    a.block_cell = B->d_block_cell;
+    a.block_cell = B->d_block_cell + B->bc_off[kShapeDefault];
+    a.shape = kShapeDefault;
     a.ntot = N;
     a.nblocks = B->nblocks;
     a.dt = dt;
@@ -1111,7 +1166,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     hipError_t e;
     if ((e = hipMemcpyAsync(B->d_cells, B->cells_h.data(), ncells * sizeof(DevCell), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(B->d_dyn, dyn.data(), ncells * sizeof(CellDyn), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
-    if ((e = hipMemcpyAsync(B->d_block_cell, B->block_cell_h.data(), B->nblocks * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(B->d_block_cell, B->block_cell_h.data(), B->block_cell_h.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemsetAsync(a.iters_total, 0, sizeof(unsigned long long), ctx->stream)) != hipSuccess) return fail(e);
     hipLaunchKernelGGL(k_seed, dim3(B->nblocks), dim3(kBlock), 0, ctx->stream, a);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
@@ -1410,13 +1465,113 @@ int gw_batch_get_iterations(gw_batch *B, uint64_t *iters)
 int gw_batch_set_launch(gw_batch *B, int32_t crus_per_thread, int32_t block, char *err, size_t errlen)
 {
     if (!B) return GW_E_INVALID_ARG;
-    if ((crus_per_thread != 0 && crus_per_thread != 1) || (block != 0 && block != kBlock)) {
-        set_err(err, errlen, "launch shape %d CRUs/thread x %d threads is not available in this build", crus_per_thread, block);
+    const bool ok_block = block == 0 || block == 64 || block == 128 || block == 256 || block == 512;
+    if ((crus_per_thread != 0 && crus_per_thread != 1) || !ok_block) {
+        set_err(err, errlen, "launch shape %d CRUs/thread x %d threads is not available (1 CRU per quad of "
+                "threads; blocks of 64, 128, 256 or 512 threads)", crus_per_thread, block);
         return GW_E_INVALID_ARG;
     }
-    B->crus_per_thread = 1;
-    B->block = kBlock;
+    const int s = block ? shape_of_block(block) : kShapeDefault;
+    if (!shape_allowed(B, s)) {
+        set_err(err, errlen, "blocks of %d threads are built for xoshiro256** without replay digests or frozen "
+                "globals only", block);
+        return GW_E_INVALID_ARG;
+    }
+    set_shape(B, s);
     return GW_OK;
+}
+
+int gw_batch_get_launch(gw_batch *B, int32_t *crus_per_thread, int32_t *block)
+{
+    if (!B) return GW_E_INVALID_ARG;
+    if (crus_per_thread) *crus_per_thread = 1;
+    if (block) *block = shape_threads(B->shape);
+    return GW_OK;
+}
+
+int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, double *ms_per_step, char *err,
+                      size_t errlen)
+{
+    if (!B || trial_steps <= 0) {
+        set_err(err, errlen, "gw_batch_autotune: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = prepare(B, err, errlen);
+    if (rc) return rc;
+    hipStream_t st = B->ctx->stream;
+    const int64_t N = B->ntot;
+    KArgs &a = B->a;
+    // everything a step changes; restored after every trial
+    struct Piece {
+        void *p;
+        size_t n;
+    };
+    std::vector<Piece> pieces = {{a.CaSS, 32 * static_cast<size_t>(N)},
+                                 {a.CaJSR, 8 * static_cast<size_t>(N)},
+                                 {a.JLCC ? static_cast<void *>(a.JLCC) : static_cast<void *>(a.JLCCf),
+                                  (a.JLCC ? 32 : 16) * static_cast<size_t>(N)},
+                                 {a.sub, 32 * static_cast<size_t>(N)},
+                                 {a.rng, 32 * static_cast<size_t>(N)},
+                                 {B->d_dyn, sizeof(CellDyn) * static_cast<size_t>(B->ncells)},
+                                 {a.iters_total, sizeof(unsigned long long)}};
+    if (a.dig) pieces.push_back({a.dig, 16 * static_cast<size_t>(N)});
+    size_t total = 0;
+    for (const Piece &q : pieces) total += (q.n + 255) & ~static_cast<size_t>(255);
+    char *snap = nullptr;
+    HIP_TRY(hipMalloc(&snap, total));
+    auto copy_all = [&](bool save) {
+        size_t off = 0;
+        for (const Piece &q : pieces) {
+            const hipError_t e = save ? hipMemcpyAsync(snap + off, q.p, q.n, hipMemcpyDeviceToDevice, st)
+                                      : hipMemcpyAsync(q.p, snap + off, q.n, hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return e;
+            off += (q.n + 255) & ~static_cast<size_t>(255);
+        }
+        return hipSuccess;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = copy_all(true);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    const int64_t steps0 = B->steps_done;
+    int best = B->shape;
+    double best_ms = 1e300;
+    for (int sh = 0; sh < kNumShapes && he == hipSuccess; ++sh) {
+        if (ms_per_step) ms_per_step[sh] = -1.0;
+        if (!shape_allowed(B, sh)) continue;
+        set_shape(B, sh);
+        launch_steps(B, B->steps_done, 2);
+        B->steps_done += 2;
+        if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
+        launch_steps(B, B->steps_done, trial_steps);
+        if ((he = hipGetLastError()) != hipSuccess) break;
+        if ((he = hipEventRecord(e1, st)) != hipSuccess) break;
+        if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+        float f = 0.f;
+        if ((he = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) break;
+        const double ms = f / static_cast<double>(trial_steps);
+        if (ms_per_step) ms_per_step[sh] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = sh;
+        }
+        he = copy_all(false);
+        B->steps_done = steps0;
+    }
+    if (he == hipSuccess) he = copy_all(false);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    B->steps_done = steps0;
+    set_shape(B, best);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    if (he != hipSuccess) {
+        set_err(err, errlen, "autotune: %s", hipGetErrorString(he));
+        return GW_E_HIP;
+    }
+    if (block_out) *block_out = shape_threads(best);
+    return check_status(B, err, errlen);
 }
 
 int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms, int64_t *launches, char *err,
@@ -1459,6 +1614,7 @@ int gw_batch_set_mode(gw_batch *B, int32_t frozen, int32_t digest)
     B->frozen = frozen != 0;
     B->a.frozen = frozen != 0;
     B->digest = digest != 0;
+    if (!shape_allowed(B, B->shape)) set_shape(B, kShapeDefault);
     return GW_OK;
 }
 

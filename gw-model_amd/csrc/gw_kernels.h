@@ -5,9 +5,18 @@
 
 namespace gw {
 
-constexpr int kBlock = 256;        // threads per SSA block
+constexpr int kBlock = 256;        // threads per block of the setup kernels (64 CRUs)
 constexpr int kWave = 64;
 constexpr int kCruPerBlock = kBlock / 4;  // one quad of lanes per CRU
+
+// SSA launch shapes: blocks of 64, 128, 256 or 512 threads (16..128 CRUs).
+// Every cell's CRU slots are padded to a multiple of the largest, so all
+// shapes address the same layout; the setup kernels use shape 2 (256).
+constexpr int kNumShapes = 4;
+constexpr int kShapeDefault = 2;
+constexpr int kSlotAlign = 128;
+__host__ __device__ constexpr int shape_threads(int s) { return 64 << s; }
+__host__ __device__ constexpr int shape_of_block(int threads) { return threads == 64 ? 0 : threads == 128 ? 1 : threads == 256 ? 2 : 3; }
 
 // Static description of one cell on the device.
 struct DevCell {
@@ -19,10 +28,10 @@ struct DevCell {
     const double *istim;   // per-step stimulus table (GW_PROTO_TABLE) or null
     int64_t istim_len;
     int64_t slot0;         // first CRU slot of this cell in the state arrays
-    int64_t blk0;          // first SSA block of this cell
+    int64_t blk0[kNumShapes];  // first SSA block of this cell, per launch shape
     uint64_t seed;
     int32_t n_cru;
-    int32_t nblk;
+    int32_t nblk[kNumShapes];  // SSA blocks of this cell, per launch shape
 };
 
 // Dynamic per-cell state, advanced by the cell's close block of each step.
@@ -53,7 +62,7 @@ struct Partial {
 struct KArgs {
     const DevCell *cells;
     CellDyn *dyn;
-    const int32_t *block_cell;
+    const int32_t *block_cell;   // cell of each SSA block of the current shape
     double *CaSS;      // [ntot][4]   (both forms: Ca accumulates in FP64)
     double *CaJSR;     // [ntot]
     double *JLCC;      // [ntot][4]   (FP64 batches)
@@ -65,7 +74,7 @@ struct KArgs {
     double *trace;     // [cap][ncells][4]
     unsigned long long *iters_total;
     int64_t ntot;
-    int64_t nblocks;
+    int64_t nblocks;     // SSA blocks of the current shape
     int64_t trace_cap;
     double dt;
     int32_t ncells;
@@ -73,7 +82,7 @@ struct KArgs {
     int32_t trace_every;
     int32_t prng;
     int32_t precision;   // GW_PREC_FP64 or GW_PREC_FP32_LOCAL
-    int32_t _pad;
+    int32_t shape;       // current SSA launch shape (0..3)
 };
 
 // CRU-local state of either precision, for the kernels outside the SSA.

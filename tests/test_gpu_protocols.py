@@ -146,3 +146,61 @@ def test_state_round_trip_and_restart(gwcapi, gpu_ctx):
     assert_states_equal(a.get_cru_state(0), s, STATE_KEYS)
     a.close()
     b.close()
+
+
+def _snapshot(b, ncells):
+    g = [x.as_array() for x in b.get_globals()]
+    s = [b.get_cru_state(c) for c in range(ncells)]
+    return g, s
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_launch_shapes_give_identical_results(gwcapi, gpu_ctx, precision):
+    """Every SSA block size gives the same bits (exact flux sums, per-CRU streams)."""
+    say(f"launch shapes, precision {precision}")
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    specs = ((37, 1), (300, 2), (1000, 3))
+    ref = None
+    for blk in (64, 128, 256, 512):
+        b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, s, None, pace) for n, s in specs], precision=precision)
+        b.set_mode(frozen=False, digest=False)
+        b.set_launch(1, blk)
+        assert b.launch() == blk
+        b.step(1500)
+        snap = _snapshot(b, len(specs))
+        b.close()
+        if ref is None:
+            ref = snap
+            continue
+        for x, y in zip(snap[0], ref[0]):
+            assert np.array_equal(x, y), blk
+        for x, y in zip(snap[1], ref[1]):
+            assert_states_equal(x, y)
+
+
+def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
+    say("autotune")
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    cells = [gwcapi.make_cell(2000, 10 + c, None, pace) for c in range(4)]
+    a = gwcapi.Batch(gpu_ctx, cells)
+    a.set_mode(frozen=False, digest=False)
+    a.step(300)
+    blk, ms = a.autotune(40)
+    say(f"  chose {blk}: {ms}")
+    assert blk in (64, 128, 256, 512) and a.launch() == blk
+    assert all(v is not None and v > 0 for v in ms.values())
+    a.step(300)
+    b = gwcapi.Batch(gpu_ctx, cells)
+    b.set_mode(frozen=False, digest=False)
+    b.step(600)
+    ga, sa = _snapshot(a, len(cells))
+    gb, sb = _snapshot(b, len(cells))
+    for x, y in zip(ga, gb):
+        assert np.array_equal(x, y)
+    for x, y in zip(sa, sb):
+        assert_states_equal(x, y)
+    t_a, n_a = a.time()
+    t_b, n_b = b.time()
+    assert n_a == n_b == 600 and t_a == t_b
+    a.close()
+    b.close()

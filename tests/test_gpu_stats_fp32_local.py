@@ -57,6 +57,9 @@ def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx):
     say("fp32-local ensemble")
     o32, g32, s32 = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps)
     a64, a32 = _stats(o64), _stats(o32)
+    for name, a in (("fp64", a64), ("fp32", a32)):
+        bad = np.flatnonzero(a["APD90"] <= 100)
+        say(f"  {name}: cells without one full beat: {bad.tolist()} APD90 {a['APD90'][bad].tolist()}")
     assert np.all(a64["APD90"] > 100) and np.all(a32["APD90"] > 100)  # every cell repolarised once
     for k in a64:
         _check_within_3se(a32[k], a64[k], k)
