@@ -73,9 +73,9 @@ def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0, autotune=
         b.step(warmup)
     tune = None
     if autotune > 0:  # trials start from the warmed-up state, which is restored afterwards
-        blk, trials = b.autotune(autotune)
-        tune = {"block": blk, "crus_per_block": blk // 4, "trial_steps": autotune,
-                "trial_ms_per_step": {str(k): v for k, v in trials.items()}}
+        (lanes, blk), trials = b.autotune(autotune)
+        tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_block": blk // lanes, "trial_steps": autotune,
+                "trial_ms_per_step": trials}
     i0 = b.iterations()
     ms, ssa_ms, launches = b.time_steps(steps, per_kernel=True)
     iters = b.iterations() - i0

@@ -139,7 +139,7 @@ def lib() -> C.CDLL:
         "gw_batch_set_launch": (C.c_int, [P, i32, i32] + E),
         "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
         "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
-        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(f64)] + E),
+        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
     }
     for name, (res, args) in sig.items():
@@ -306,21 +306,24 @@ class Batch:
         self.L.gw_batch_get_iterations(self.h, C.byref(v))
         return v.value
 
-    def set_launch(self, crus_per_thread: int = 0, block: int = 0):
-        _call(self.L.gw_batch_set_launch, self.h, crus_per_thread, block)
+    SHAPES = ((4, 64), (4, 128), (4, 256), (4, 512), (1, 64), (1, 128))
 
-    def launch(self) -> int:
-        """Threads per block of the current SSA launch shape."""
-        cpt, blk = i32(), i32()
-        self.L.gw_batch_get_launch(self.h, C.byref(cpt), C.byref(blk))
-        return blk.value
+    def set_launch(self, lanes_per_cru: int = 0, block: int = 0):
+        _call(self.L.gw_batch_set_launch, self.h, lanes_per_cru, block)
 
-    def autotune(self, trial_steps: int = 200) -> tuple[int, dict]:
-        """gw_batch_autotune: pick the fastest SSA block size on this batch (its state is left unchanged)."""
-        blk = i32()
-        ms = (f64 * 4)()
-        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(blk), ms)
-        return blk.value, {64 << k: (ms[k] if ms[k] >= 0 else None) for k in range(4)}
+    def launch(self) -> tuple[int, int]:
+        """(lanes per CRU, threads per block) of the current SSA launch shape."""
+        lanes, blk = i32(), i32()
+        self.L.gw_batch_get_launch(self.h, C.byref(lanes), C.byref(blk))
+        return lanes.value, blk.value
+
+    def autotune(self, trial_steps: int = 200) -> tuple[tuple[int, int], dict]:
+        """gw_batch_autotune: pick the fastest SSA launch shape on this batch (its state is left unchanged)."""
+        lanes, blk = i32(), i32()
+        ms = (f64 * 6)()
+        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(lanes), C.byref(blk), ms)
+        return (lanes.value, blk.value), {f"{l}x{b}": (ms[k] if ms[k] >= 0 else None)
+                                          for k, (l, b) in enumerate(self.SHAPES)}
 
     def time_steps(self, nsteps: int, per_kernel: bool = False):
         """(total ms, SSA-kernel ms or None, SSA launches) of nsteps Euler steps."""

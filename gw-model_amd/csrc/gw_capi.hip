@@ -527,6 +527,117 @@ __global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64
     ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, ssa_step, 1, T, acc);
 }
 
+// The SSA of the 64 or 128 CRUs of block b with one lane per CRU (FP32-local
+// variant). Per-wave sums of the CRU outputs are formed in FP64 by a fixed
+// xor butterfly (the same value in every lane), then added exactly across
+// waves and blocks as in ssa_block.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+template <int PRNG, int BLOCK>
+__device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t step0, LccTableF &T, BlockAcc &acc)
+{
+    constexpr int kShape = (BLOCK == 64) ? 2 : 3;   // the block -> cell map of 64 or 128 CRUs per block
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const int ks = static_cast<int>(step0 & 1);
+    const DevCell &C = a.cells[cell];
+    const CellDyn &D = a.dyn[cell];
+    const SsaParamsF &sp = C.spf;
+    const StepConstsF &kk = D.kf[ks];
+    load_lcc_table(kk, T);
+    acc_zero(acc);
+    __syncthreads();
+    const int64_t i = (b - C.blk0[kShape]) * BLOCK + tid;
+    const bool active = (i < C.n_cru) && (D.status == 0);
+    LaneOut o{};
+    if (active) {
+        const int64_t slot = C.slot0 + i;
+        double ca[4];
+        float jl[4];
+        uint64_t w[4];
+        CruQ q;
+        const double2 *pc = reinterpret_cast<const double2 *>(a.CaSS + slot * 4);
+        const double2 c01 = pc[0], c23 = pc[1];
+        ca[0] = c01.x; ca[1] = c01.y; ca[2] = c23.x; ca[3] = c23.y;
+        const float4 j4 = *reinterpret_cast<const float4 *>(a.JLCCf + slot * 4);
+        jl[0] = j4.x; jl[1] = j4.y; jl[2] = j4.z; jl[3] = j4.w;
+        const ulonglong2 *pw = reinterpret_cast<const ulonglong2 *>(a.sub + slot * 4);
+        const ulonglong2 w01 = pw[0], w23 = pw[1];
+        w[0] = w01.x; w[1] = w01.y; w[2] = w23.x; w[3] = w23.y;
+        const ulonglong2 *pr = reinterpret_cast<const ulonglong2 *>(a.rng + slot * 4);
+        const ulonglong2 r01 = pr[0], r23 = pr[1];
+        q.r0 = r01.x; q.r1 = r01.y; q.r2 = r23.x; q.r3 = r23.y;
+        q.jsr = a.CaJSR[slot];
+        o = ssa_step_lane<PRNG>(ca, jl, w, q, sp, kk, T, static_cast<float>(a.dt));
+        double2 *qc = reinterpret_cast<double2 *>(a.CaSS + slot * 4);
+        qc[0] = make_double2(ca[0], ca[1]);
+        qc[1] = make_double2(ca[2], ca[3]);
+        *reinterpret_cast<float4 *>(a.JLCCf + slot * 4) = make_float4(jl[0], jl[1], jl[2], jl[3]);
+        ulonglong2 *qw = reinterpret_cast<ulonglong2 *>(a.sub + slot * 4);
+        qw[0] = make_ulonglong2(w[0], w[1]);
+        qw[1] = make_ulonglong2(w[2], w[3]);
+        ulonglong2 *qr = reinterpret_cast<ulonglong2 *>(a.rng + slot * 4);
+        qr[0] = make_ulonglong2(q.r0, q.r1);
+        qr[1] = make_ulonglong2(q.r2, q.r3);
+        a.CaJSR[slot] = q.jsr;
+    }
+    const double sx = wave_sum(o.Jxfer), st = wave_sum(o.Jtr), sl = wave_sum(o.JLCC), sr = wave_sum(o.Jrel);
+    const int n_it = wave_sum(static_cast<int>(o.iters)), n_open = wave_sum(o.open), n_cl = wave_sum(o.clch);
+    const int k = lane & 3;
+    const double v = k == 0 ? sx : k == 1 ? st : k == 2 ? sl : sr;
+    Fx f;
+    fx_zero(f);
+    const int fl = fx_add(f, v);
+    const bool bad = (__ballot(o.err != 0) | __ballot(lane < 3 && fl != 0)) != 0;
+    if (lane < 4) {
+        acc_add_fx(acc, lane, f);
+        const int64_t iv = lane == 0 ? n_it : lane == 1 ? n_open : lane == 2 ? n_cl : (bad ? 1 : 0);
+        atomicAdd(&acc.ints[lane], static_cast<unsigned long long>(iv));
+    }
+    __syncthreads();
+    if (tid == 0) {
+        Partial p;
+        p.jxfer = acc_fx(acc, 0);
+        p.jtr = acc_fx(acc, 1);
+        p.jlcc = acc_fx(acc, 2);
+        p.jrel = acc_fx(acc, 3);
+        p.iters = acc.ints[0];
+        p.open = static_cast<int64_t>(acc.ints[1]);
+        p.clch = static_cast<int64_t>(acc.ints[2]);
+        p.err = acc.ints[3] ? 1 : 0;
+        p._pad = 0;
+        a.partials[ks * a.nblocks + b] = p;
+    }
+}
+
+// One lane per CRU needs about 100-128 VGPRs; 4 waves per SIMD keep it free of spills
+// and still hold 256 CRUs per SIMD.
+template <int PRNG, int BLOCK>
+__global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_step, int32_t nclose)
+{
+    __shared__ LccTableF T;
+    __shared__ Partial red[kWave];
+    __shared__ BlockAcc acc;
+    const int bid = blockIdx.x;
+    if (bid < nclose) {
+        close_cell<kWave>(a, bid, red);
+        return;
+    }
+    ssa_block_lane<PRNG, BLOCK>(a, bid - nclose, ssa_step, T, acc);
+}
+
 // Frozen-globals replay: nsteps SSA steps in one launch.
 template <int PRNG, bool DIGEST, class R>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_ssa(KArgs a, int64_t step0, int64_t nsteps)
@@ -670,7 +781,8 @@ struct gw_batch {
     int64_t steps_done = 0;
     int32_t trace_every = 0;
     int64_t trace_cap = 0;
-    int32_t shape = kShapeDefault;
+    int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
+    int32_t lanes = 4;               // lanes per CRU: 4 (quad) or 1 (FP32-local only)
     int64_t nblocks_shape[kNumShapes] = {0, 0, 0, 0};
     int64_t bc_off[kNumShapes] = {0, 0, 0, 0};   // offset of each shape's block -> cell map
     int64_t max_nblocks = 0;
@@ -716,8 +828,9 @@ int launch_cfg_check(gw_batch *B, char *err, size_t errlen)
     return GW_OK;
 }
 
-void set_shape(gw_batch *B, int s)
+void set_shape(gw_batch *B, int s, int lanes = 4)
 {
+    B->lanes = lanes;
     B->shape = s;
     B->nblocks = B->nblocks_shape[s];
     B->a.shape = s;
@@ -727,11 +840,19 @@ void set_shape(gw_batch *B, int s)
 
 // Shapes other than the default are built for the fast path only: FP64
 // without replay digests or FP32-local, xoshiro256**, free-running globals.
-bool shape_allowed(const gw_batch *B, int s)
+bool shape_allowed(const gw_batch *B, int s, int lanes = 4)
 {
+    if (lanes == 1)   // one lane per CRU: FP32-local, 64 or 128 CRUs per block
+        return B->precision == GW_PREC_FP32_LOCAL && B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen && s >= 2;
     return s == kShapeDefault || (B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen &&
                                   (B->precision == GW_PREC_FP32_LOCAL || !B->digest));
 }
+
+// The launch configurations the autotuner tries: (lanes per CRU, threads per block).
+constexpr int kNumConfigs = 6;
+constexpr int kConfigLanes[kNumConfigs] = {4, 4, 4, 4, 1, 1};
+constexpr int kConfigThreads[kNumConfigs] = {64, 128, 256, 512, 64, 128};
+int config_shape(int c) { return kConfigLanes[c] == 4 ? shape_of_block(kConfigThreads[c]) : (kConfigThreads[c] == 64 ? 2 : 3); }
 
 int prepare(gw_batch *B, char *err, size_t errlen)
 {
@@ -769,24 +890,35 @@ void launch_frozen_steps(gw_batch *B, int64_t step0, int64_t n)
     }
 }
 
+// Grid of one launch: the closes, then one block per SSA tile.
+unsigned step_grid(const gw_batch *B, int64_t ssa_step, int32_t nclose)
+{
+    return static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
+}
+
 // One launch of k_step: SSA(ssa_step) (none if ssa_step < 0) beside the
 // closes of the previous step (when nclose > 0). When e0/e1 are given, the
 // dispatch itself records them (its start and end, no extra queue packets).
 template <int PRNG, int BLOCK>
 void launch_k_step_shape(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
 {
-    const unsigned grid = static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
     if (B->precision == GW_PREC_FP32_LOCAL) {
-        hipExtLaunchKernelGGL((k_step<PRNG, false, float, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+        const auto k = k_step<PRNG, false, float, BLOCK>;
+        const unsigned grid = step_grid(B, ssa_step, nclose);
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
         return;
     }
     if constexpr (BLOCK == kBlock) {
         if (B->digest) {
-            hipExtLaunchKernelGGL((k_step<PRNG, true, double, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+            const auto k = k_step<PRNG, true, double, BLOCK>;
+            const unsigned grid = step_grid(B, ssa_step, nclose);
+            hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
             return;
         }
     }
-    hipExtLaunchKernelGGL((k_step<PRNG, false, double, BLOCK>), dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+    const auto k = k_step<PRNG, false, double, BLOCK>;
+    const unsigned grid = step_grid(B, ssa_step, nclose);
+    hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
 }
 
 // Shapes other than the default are built for xoshiro256** only (the engine
@@ -795,6 +927,18 @@ template <int PRNG>
 void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
 {
     if constexpr (PRNG == GW_PRNG_XOSHIRO256SS) {
+        if (B->lanes == 1) {
+            if (B->shape == 2) {
+                const auto k = k_step_lane<PRNG, 64>;
+                const unsigned grid = step_grid(B, ssa_step, nclose);
+                hipExtLaunchKernelGGL(k, dim3(grid), dim3(64), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+            } else {
+                const auto k = k_step_lane<PRNG, 128>;
+                const unsigned grid = step_grid(B, ssa_step, nclose);
+                hipExtLaunchKernelGGL(k, dim3(grid), dim3(128), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+            }
+            return;
+        }
         switch (B->shape) {
         case 0: launch_k_step_shape<PRNG, 64>(B, ssa_step, nclose, e0, e1); return;
         case 1: launch_k_step_shape<PRNG, 128>(B, ssa_step, nclose, e0, e1); return;
@@ -1462,35 +1606,39 @@ int gw_batch_get_iterations(gw_batch *B, uint64_t *iters)
     return GW_OK;
 }
 
-int gw_batch_set_launch(gw_batch *B, int32_t crus_per_thread, int32_t block, char *err, size_t errlen)
+int gw_batch_set_launch(gw_batch *B, int32_t lanes_per_cru, int32_t block, char *err, size_t errlen)
 {
     if (!B) return GW_E_INVALID_ARG;
-    const bool ok_block = block == 0 || block == 64 || block == 128 || block == 256 || block == 512;
-    if ((crus_per_thread != 0 && crus_per_thread != 1) || !ok_block) {
-        set_err(err, errlen, "launch shape %d CRUs/thread x %d threads is not available (1 CRU per quad of "
-                "threads; blocks of 64, 128, 256 or 512 threads)", crus_per_thread, block);
+    const int lanes = lanes_per_cru ? lanes_per_cru : 4;
+    const int threads = block ? block : (lanes == 4 ? kBlock : 64);
+    int c = -1;
+    for (int k = 0; k < kNumConfigs; ++k)
+        if (kConfigLanes[k] == lanes && kConfigThreads[k] == threads) c = k;
+    if (c < 0) {
+        set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads is not available (4 lanes: 64, 128, 256 "
+                "or 512 threads; 1 lane: 64 or 128 threads)", lanes_per_cru, block);
         return GW_E_INVALID_ARG;
     }
-    const int s = block ? shape_of_block(block) : kShapeDefault;
-    if (!shape_allowed(B, s)) {
-        set_err(err, errlen, "blocks of %d threads are built for xoshiro256** without replay digests or frozen "
-                "globals only", block);
+    if (!shape_allowed(B, config_shape(c), lanes)) {
+        set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads is not available for this batch (other "
+                "shapes than 4 x 256 need xoshiro256** without replay digests or frozen globals; 1 lane per CRU needs "
+                "FP32-local)", lanes, threads);
         return GW_E_INVALID_ARG;
     }
-    set_shape(B, s);
+    set_shape(B, config_shape(c), lanes);
     return GW_OK;
 }
 
-int gw_batch_get_launch(gw_batch *B, int32_t *crus_per_thread, int32_t *block)
+int gw_batch_get_launch(gw_batch *B, int32_t *lanes_per_cru, int32_t *block)
 {
     if (!B) return GW_E_INVALID_ARG;
-    if (crus_per_thread) *crus_per_thread = 1;
-    if (block) *block = shape_threads(B->shape);
+    if (lanes_per_cru) *lanes_per_cru = B->lanes;
+    if (block) *block = B->lanes == 4 ? shape_threads(B->shape) : (B->shape == 2 ? 64 : 128);
     return GW_OK;
 }
 
-int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, double *ms_per_step, char *err,
-                      size_t errlen)
+int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out, double *ms_per_step,
+                      char *err, size_t errlen)
 {
     if (!B || trial_steps <= 0) {
         set_err(err, errlen, "gw_batch_autotune: bad arguments");
@@ -1535,12 +1683,12 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, doub
     if (he == hipSuccess) he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
     const int64_t steps0 = B->steps_done;
-    int best = B->shape;
+    int best_shape = B->shape, best_lanes = B->lanes;
     double best_ms = 1e300;
-    for (int sh = 0; sh < kNumShapes && he == hipSuccess; ++sh) {
-        if (ms_per_step) ms_per_step[sh] = -1.0;
-        if (!shape_allowed(B, sh)) continue;
-        set_shape(B, sh);
+    for (int c = 0; c < kNumConfigs && he == hipSuccess; ++c) {
+        if (ms_per_step) ms_per_step[c] = -1.0;
+        if (!shape_allowed(B, config_shape(c), kConfigLanes[c])) continue;
+        set_shape(B, config_shape(c), kConfigLanes[c]);
         launch_steps(B, B->steps_done, 2);
         B->steps_done += 2;
         if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
@@ -1551,10 +1699,11 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, doub
         float f = 0.f;
         if ((he = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) break;
         const double ms = f / static_cast<double>(trial_steps);
-        if (ms_per_step) ms_per_step[sh] = ms;
+        if (ms_per_step) ms_per_step[c] = ms;
         if (ms < best_ms) {
             best_ms = ms;
-            best = sh;
+            best_shape = config_shape(c);
+            best_lanes = kConfigLanes[c];
         }
         he = copy_all(false);
         B->steps_done = steps0;
@@ -1562,7 +1711,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, doub
     if (he == hipSuccess) he = copy_all(false);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     B->steps_done = steps0;
-    set_shape(B, best);
+    set_shape(B, best_shape, best_lanes);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(snap);
@@ -1570,7 +1719,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *block_out, doub
         set_err(err, errlen, "autotune: %s", hipGetErrorString(he));
         return GW_E_HIP;
     }
-    if (block_out) *block_out = shape_threads(best);
+    gw_batch_get_launch(B, lanes_out, block_out);
     return check_status(B, err, errlen);
 }
 
@@ -1614,7 +1763,7 @@ int gw_batch_set_mode(gw_batch *B, int32_t frozen, int32_t digest)
     B->frozen = frozen != 0;
     B->a.frozen = frozen != 0;
     B->digest = digest != 0;
-    if (!shape_allowed(B, B->shape)) set_shape(B, kShapeDefault);
+    if (!shape_allowed(B, B->shape, B->lanes)) set_shape(B, kShapeDefault);
     return GW_OK;
 }
 

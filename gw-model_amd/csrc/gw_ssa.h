@@ -459,4 +459,190 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// One lane per CRU (FP32-local variant). The lane visits its four subunits in
+// order, as SSA_single_CRU itself does (SSA.tpp:37-99, 262-283); the sampled
+// subunit's values are read and written through selects, so no register array
+// is indexed with a run-time index.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T sel4(const T (&v)[4], int k) { return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3]; }
+template <class T>
+__device__ __forceinline__ void put4(T (&v)[4], int k, T x)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (k == j) v[j] = x;
+}
+
+struct LaneOut {
+    double Jxfer, Jtr, JLCC, Jrel;   // per-CRU sums in subunit order
+    int open, clch;
+    uint32_t iters;
+    int err;
+};
+
+template <int PRNG>
+__device__ LaneOut ssa_step_lane(double (&ca)[4], float (&jl)[4], uint64_t (&w)[4], CruQ &q, const SsaParamsF &P,
+                                 const StepConstsF &K, const LccTableF &T, float dt)
+{
+    using R = float;
+    LaneOut out;
+    out.iters = 0;
+    out.err = 0;
+    const double Cai = K.Cai_d, CaNSR = K.CaNSR_d;
+    // copy_from_CRUState (SSA.tpp:10-33)
+    int open[4];
+    R Jrel[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        open[j] = ryr_get(w[j], 2) + ryr_get(w[j], 3);
+        Jrel[j] = P.rRyR * static_cast<R>(open[j]) * static_cast<R>(q.jsr - ca[j]);
+    }
+    R t = R(0), Jtr = R(0);
+    R Jxfer[4] = {R(0), R(0), R(0), R(0)};
+    while (true) {
+        out.iters++;
+        // update_fluxes + update_rates (SSA.tpp:37-56)
+        double oldA[4];
+        R old[4], s[4];
+        uint64_t w_pre[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            oldA[j] = ca[j];
+            old[j] = static_cast<R>(oldA[j]);
+            w_pre[j] = w[j];
+        }
+        Jtr = P.rtr * static_cast<R>(CaNSR - q.jsr);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, w[j], old[j]);
+        const R total = s[0] + s[1] + s[2] + s[3];
+        const R jump = ldiv(lneglog(qdraw<PRNG, R>(q)), total);
+        const bool last = !(t + jump < dt);
+        const R h = last ? dt - t : jump;
+        // update_CaSS (SSA.tpp:87-99): subunits in order, re-splits drawn as they come
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const R Jiss = P.riss * static_cast<R>(oldA[(j + 1) & 3] + oldA[(j + 3) & 3] - 2 * oldA[j]);
+            Jxfer[j] = P.rxfer * static_cast<R>(oldA[j] - Cai);
+            const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old[j])) + ldiv(P.BSL_const, sq(P.KBSL + old[j])));
+            const R dCaSS = h * (jl[j] + Jrel[j] - Jxfer[j] + Jiss) * betaSS;
+            const double tmp = oldA[j] + static_cast<double>(dCaSS);
+            const int split = (oldA[j] > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((oldA[j] > 0.03685 && tmp <= 0.03685) ? 2 : 0);
+            if (split) {
+                // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84)
+                const int a_ = (split == 1) ? 4 : 2;
+                const R pr = (split == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(old[j])) : ldiv(P.k43, P.k43 + P.k34 * sq(old[j]));
+                const int n = ryr_get(w[j], a_) + ryr_get(w[j], a_ + 1);
+                int X = 0;
+                for (int i = 0; i < n; ++i)
+                    if (qdraw<PRNG, R>(q) < pr) X++;
+                w[j] = ryr_set(ryr_set(w[j], a_, (n > 0) ? X : 0), a_ + 1, (n > 0) ? n - X : 0);
+            }
+            ca[j] = tmp;
+        }
+        // update_CaJSR (SSA.hpp:67-71)
+        {
+            const R Jsum = Jrel[0] + Jrel[1] + Jrel[2] + Jrel[3];
+            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(q.jsr))));
+            q.jsr += static_cast<double>(h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
+        }
+        if (last) break;
+        if (out.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
+            out.err = 1;
+            break;
+        }
+        t += jump;
+        // sample_weights over the subunits (common.hpp:49-59)
+        int js;
+        {
+            const R u = qdraw<PRNG, R>(q) * total;
+            R cum = s[0];
+            js = 0;
+            if (cum < u) { js = 1; cum += s[1];
+                if (cum < u) { js = 2; cum += s[2];
+                    if (cum < u) { js = 3; } } }
+        }
+        // sample_new_state (SSA.tpp:262-283) on subunit js
+        const R u3 = qdraw<PRNG, R>(q);
+        uint64_t wj = sel4(w, js);
+        const uint64_t wpj = sel4(w_pre, js);
+        const R oldj = sel4(old, js);
+        const R sj = sel4(s, js);
+        const R caj = static_cast<R>(sel4(ca, js));
+        R jlj = sel4(jl, js);
+        R Jrelj = sel4(Jrel, js);
+        int openj = sel4(open, js);
+        const int L = sub_lcc(wj);
+        const int y = sub_y(wj);
+        const int cl = sub_cl(wj);
+        const R sum_LCC = T.P[L] + T.Q[L] * oldj;
+        const R inact = y ? K.inact1 : K.inact0;
+        const R clr = cl ? P.kbClCh : P.kfClCh * oldj;
+        const R sum_RyR = sj - (sum_LCC + inact + clr);
+        const R u = u3 * sj;
+        const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
+        const R u4 = (cls == 0 || cls == 2) ? qdraw<PRNG, R>(q) : R(0);
+        if (cls == 0) {
+            // sample_LCC (SSA.tpp:103-197)
+            const R r0 = T.R0[L];
+            const R r1 = (L == 1) ? T.Q[L] * oldj : T.R1[L];
+            const R uu = u4 * sum_LCC;
+            R cum = r0;
+            int tr = 0;
+            if (cum < uu) { tr = 1; cum += r1;
+                if (cum < uu) { tr = 2; } }
+            const int Ln = lcc_next(L, tr);
+            if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
+                jlj = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj) : R(0);
+            else if (L == 6 || L == 12)
+                jlj = R(0);
+            wj = sub_set_lcc(wj, Ln);
+        } else if (cls == 1) {
+            const int yn = 1 - y;
+            wj = sub_flip_y(wj);
+            if (yn == 0) jlj = R(0);
+            else if (L == 6 || L == 12) jlj = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj);
+        } else if (cls == 2) {
+            // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
+            const R uu = u4 * sum_RyR;
+            int tr = 0;
+            R cum = R(0);
+            bool done = false;
+            ryr_rates(P, wpj, oldj, [&](int k, R r) {
+                if (done) return;
+                if (k == 0) { cum = r; }
+                else { if (cum < uu) { tr = k; cum += r; } else done = true; }
+            });
+            const int from = (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
+                           : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
+            const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
+                         : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
+            wj = ryr_add(ryr_add(wj, from, -1), to, +1);
+            if (tr == 1 || tr == 10) openj += 1;
+            else if (tr == 4 || tr == 7) openj -= 1;
+            Jrelj = P.rRyR * static_cast<R>(openj) * static_cast<R>(q.jsr - sel4(ca, js));
+        } else {
+            wj = sub_flip_cl(wj);
+        }
+        put4(w, js, wj);
+        put4(jl, js, jlj);
+        put4(Jrel, js, Jrelj);
+        put4(open, js, openj);
+    }
+    // outputs of update_CRUstate_from_temp (GW.tpp:47-62), summed in subunit order in FP64
+    out.Jxfer = ((static_cast<double>(Jxfer[0]) + Jxfer[1]) + Jxfer[2]) + Jxfer[3];
+    out.Jtr = static_cast<double>(Jtr);
+    out.JLCC = ((static_cast<double>(jl[0]) + jl[1]) + jl[2]) + jl[3];
+    out.Jrel = ((static_cast<double>(Jrel[0]) + Jrel[1]) + Jrel[2]) + Jrel[3];
+    out.open = 0;
+    out.clch = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        out.open += ryr_get(w[j], 2) + ryr_get(w[j], 3);
+        out.clch += sub_cl(w[j]);
+    }
+    return out;
+}
+
 }  // namespace gw

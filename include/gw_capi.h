@@ -279,24 +279,26 @@ int gw_batch_set_mode(gw_batch *b, int32_t frozen, int32_t digest);
 /* Total SSA loop iterations executed (diagnostic for the flop count per CRU-step). */
 int gw_batch_get_iterations(gw_batch *b, uint64_t *iters);
 /*
- * Launch shape of the SSA kernel: CRUs per quad of threads (1; 0 = default)
- * and threads per block (64, 128, 256 or 512; 0 = 256). Results do not depend
- * on it (the flux sums are exact). Shapes other than 256 threads exist for
- * xoshiro256** batches without replay digests or frozen globals.
- * No reference counterpart (the reference has one OpenMP loop, GW.tpp:223-233).
+ * Launch shape of the SSA kernel: lanes per CRU (4: a quad of lanes per CRU,
+ * one subunit per lane; 1: one lane per CRU, FP32-local batches only; 0 = 4)
+ * and threads per block (4 lanes: 64, 128, 256 or 512; 1 lane: 64 or 128;
+ * 0 = 256 or 64). FP64 results do not depend on it (exact flux sums). Shapes
+ * other than 4 x 256 exist for xoshiro256** batches without replay digests or
+ * frozen globals. No reference counterpart (the reference has one OpenMP
+ * loop over CRUs, src/includes/GW.tpp:223-233).
  */
-int gw_batch_set_launch(gw_batch *b, int32_t crus_per_thread, int32_t block,
+int gw_batch_set_launch(gw_batch *b, int32_t lanes_per_cru, int32_t block,
                         char *err, size_t errlen);
-int gw_batch_get_launch(gw_batch *b, int32_t *crus_per_thread, int32_t *block);
+int gw_batch_get_launch(gw_batch *b, int32_t *lanes_per_cru, int32_t *block);
 /*
  * Autotuner: times trial_steps Euler steps of the batch in every available
- * launch shape, starting each trial from the same state (the batch state is
- * saved before and restored after), and keeps the fastest. *block_out gets its
- * threads per block; ms_per_step[4] (may be NULL) the time per step of blocks
- * of 64, 128, 256 and 512 threads (-1 where a shape is not available).
+ * launch shape, each trial starting from the same state (the batch state is
+ * saved before and restored after), and keeps the fastest. ms_per_step[6]
+ * (may be NULL) receives the time per step of the shapes 4x64, 4x128, 4x256,
+ * 4x512, 1x64 and 1x128 (lanes per CRU x threads; -1 where not available).
  */
-int gw_batch_autotune(gw_batch *b, int64_t trial_steps, int32_t *block_out, double *ms_per_step,
-                      char *err, size_t errlen);
+int gw_batch_autotune(gw_batch *b, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out,
+                      double *ms_per_step, char *err, size_t errlen);
 /*
  * Time nsteps Euler steps with HIP events on the batch's stream: *ms is the
  * wall time of the whole sequence, *launches the number of SSA dispatches.

@@ -164,8 +164,8 @@ def test_launch_shapes_give_identical_results(gwcapi, gpu_ctx, precision):
     for blk in (64, 128, 256, 512):
         b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, s, None, pace) for n, s in specs], precision=precision)
         b.set_mode(frozen=False, digest=False)
-        b.set_launch(1, blk)
-        assert b.launch() == blk
+        b.set_launch(4, blk)
+        assert b.launch() == (4, blk)
         b.step(1500)
         snap = _snapshot(b, len(specs))
         b.close()
@@ -185,10 +185,11 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
     a = gwcapi.Batch(gpu_ctx, cells)
     a.set_mode(frozen=False, digest=False)
     a.step(300)
-    blk, ms = a.autotune(40)
-    say(f"  chose {blk}: {ms}")
-    assert blk in (64, 128, 256, 512) and a.launch() == blk
-    assert all(v is not None and v > 0 for v in ms.values())
+    shape, ms = a.autotune(40)
+    say(f"  chose {shape}: {ms}")
+    assert shape in ((4, 64), (4, 128), (4, 256), (4, 512)) and a.launch() == shape
+    assert all(ms[k] is not None and ms[k] > 0 for k in ("4x64", "4x128", "4x256", "4x512"))
+    assert ms["1x64"] is None and ms["1x128"] is None  # one lane per CRU is FP32-local only
     a.step(300)
     b = gwcapi.Batch(gpu_ctx, cells)
     b.set_mode(frozen=False, digest=False)

@@ -19,9 +19,12 @@ pytestmark = pytest.mark.gpu
 PACE = {"kind": 1, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
 
 
-def _run(gwcapi, ctx, precision, cells, steps):
+def _run(gwcapi, ctx, precision, cells, steps, launch=None):
     b = gwcapi.Batch(ctx, cells, precision=precision)
     b.set_mode(frozen=False, digest=False)
+    if launch:
+        b.set_launch(*launch)
+        assert b.launch() == launch
     b.step(steps)
     obs = b.obs()
     g = b.get_globals()
@@ -47,15 +50,22 @@ def _check_within_3se(a, b, name):
     return diff / b.mean()
 
 
-def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx):
+_FP64_ENSEMBLE = {}
+
+
+@pytest.mark.parametrize("launch", [(4, 256), (1, 64)], ids=["quad", "lane"])
+def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx, launch):
     """64 seeds, one paced beat (420 ms): APD90, peak Cai, peak ICaL and mean
-    RyR open fraction agree within 3 SE."""
+    RyR open fraction agree within 3 SE, for a quad of lanes per CRU and for
+    one lane per CRU."""
     seeds, n, steps = 64, 1000, 420_000
     cells = [gwcapi.make_cell(n, 500 + s, None, PACE) for s in range(seeds)]
-    say(f"fp64 ensemble {seeds} x {n} CRUs x {steps} steps")
-    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps)
-    say("fp32-local ensemble")
-    o32, g32, s32 = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps)
+    if "o64" not in _FP64_ENSEMBLE:
+        say(f"fp64 ensemble {seeds} x {n} CRUs x {steps} steps")
+        _FP64_ENSEMBLE["o64"] = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps)[0]
+    o64 = _FP64_ENSEMBLE["o64"]
+    say(f"fp32-local ensemble, launch {launch}")
+    o32, g32, s32 = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps, launch)
     a64, a32 = _stats(o64), _stats(o32)
     for name, a in (("fp64", a64), ("fp32", a32)):
         bad = np.flatnonzero(a["APD90"] <= 100)
@@ -66,11 +76,12 @@ def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx):
     assert all(np.isfinite(x.V) and x.V < -80 for x in g32)
 
 
-def test_fp32_local_state_invariants(gwcapi, gpu_ctx):
+@pytest.mark.parametrize("launch", [(4, 256), (1, 64), (1, 128)], ids=["quad", "lane64", "lane128"])
+def test_fp32_local_state_invariants(gwcapi, gpu_ctx, launch):
     """Channel states stay exact integers with conserved RyR counts; Ca stays finite and positive."""
     n = 4000
-    say("fp32-local invariants")
-    _, g, s = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, [gwcapi.make_cell(n, 9, None, PACE)], 30_000)
+    say(f"fp32-local invariants, launch {launch}")
+    _, g, s = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, [gwcapi.make_cell(n, 9, None, PACE)], 30_000, launch)
     R = s["RyR"]
     assert np.all(R.sum(axis=2) == 5)
     assert np.all(R[..., 2] + R[..., 3] >= 0) and np.all(R[..., 4] + R[..., 5] >= 0)
