@@ -74,8 +74,8 @@ def gpu_bench(capi, cells, crus, steps, warmup, seed0=42, precision=0, autotune=
     tune = None
     if autotune > 0:  # trials start from the warmed-up state, which is restored afterwards
         (lanes, blk), trials = b.autotune(autotune)
-        tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_block": blk // lanes, "trial_steps": autotune,
-                "trial_ms_per_step": trials}
+        tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_block": blk // lanes,
+                "graph_steps": b.graph_steps, "trial_steps": autotune, "trial_ms_per_step": trials}
     i0 = b.iterations()
     ms, ssa_ms, launches = b.time_steps(steps, per_kernel=True)
     iters = b.iterations() - i0

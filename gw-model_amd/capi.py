@@ -139,7 +139,8 @@ def lib() -> C.CDLL:
         "gw_batch_set_launch": (C.c_int, [P, i32, i32] + E),
         "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
         "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
-        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)] + E),
+        "gw_batch_set_graph": (C.c_int, [P, i32] + E),
+        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
     }
     for name, (res, args) in sig.items():
@@ -311,6 +312,10 @@ class Batch:
     def set_launch(self, lanes_per_cru: int = 0, block: int = 0):
         _call(self.L.gw_batch_set_launch, self.h, lanes_per_cru, block)
 
+    def set_graph(self, steps_per_graph: int):
+        """Replay step launches as captured HIP graphs of this many steps (0: plain launches)."""
+        _call(self.L.gw_batch_set_graph, self.h, int(steps_per_graph))
+
     def launch(self) -> tuple[int, int]:
         """(lanes per CRU, threads per block) of the current SSA launch shape."""
         lanes, blk = i32(), i32()
@@ -318,12 +323,16 @@ class Batch:
         return lanes.value, blk.value
 
     def autotune(self, trial_steps: int = 200) -> tuple[tuple[int, int], dict]:
-        """gw_batch_autotune: pick the fastest SSA launch shape on this batch (its state is left unchanged)."""
-        lanes, blk = i32(), i32()
-        ms = (f64 * 6)()
-        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(lanes), C.byref(blk), ms)
-        return (lanes.value, blk.value), {f"{l}x{b}": (ms[k] if ms[k] >= 0 else None)
-                                          for k, (l, b) in enumerate(self.SHAPES)}
+        """gw_batch_autotune: pick the fastest SSA launch shape, with or without
+        graph replay, on this batch (its state is left unchanged). Returns the
+        shape and the trial times per step; `graph` is the chosen graph size."""
+        lanes, blk, graph = i32(), i32(), i32()
+        ms = (f64 * 7)()
+        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(lanes), C.byref(blk), C.byref(graph), ms)
+        trials = {f"{l}x{b}": (ms[k] if ms[k] >= 0 else None) for k, (l, b) in enumerate(self.SHAPES)}
+        trials["graph64"] = ms[6] if ms[6] >= 0 else None
+        self.graph_steps = graph.value
+        return (lanes.value, blk.value), trials
 
     def time_steps(self, nsteps: int, per_kernel: bool = False):
         """(total ms, SSA-kernel ms or None, SSA launches) of nsteps Euler steps."""

@@ -513,8 +513,10 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *re
 // spills to scratch, which it can afford).
 template <class R> constexpr int kWavesPerSimd = sizeof(R) == 4 ? 6 : 4;
 
+// ssa_step is relative to *base when base is given (replayed graphs of steps).
 template <int PRNG, bool DIGEST, class R, int BLOCK>
-__global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64_t ssa_step, int32_t nclose)
+__global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64_t ssa_step, int32_t nclose,
+                                                                   const int64_t *base)
 {
     __shared__ LccTableT<R> T;
     __shared__ Partial red[kWave];
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64
         close_cell<kWave>(a, bid, red);
         return;
     }
-    ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, ssa_step, 1, T, acc);
+    ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, 1, T, acc);
 }
 
 // The SSA of the 64 or 128 CRUs of block b with one lane per CRU (FP32-local
@@ -625,7 +627,7 @@ __device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t st
 // One lane per CRU needs about 100-128 VGPRs; 4 waves per SIMD keep it free of spills
 // and still hold 256 CRUs per SIMD.
 template <int PRNG, int BLOCK>
-__global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_step, int32_t nclose)
+__global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_step, int32_t nclose, const int64_t *base)
 {
     __shared__ LccTableF T;
     __shared__ Partial red[kWave];
@@ -635,8 +637,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_ste
         close_cell<kWave>(a, bid, red);
         return;
     }
-    ssa_block_lane<PRNG, BLOCK>(a, bid - nclose, ssa_step, T, acc);
+    ssa_block_lane<PRNG, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc);
 }
+
+// Graph bookkeeping: the step index a replayed chunk starts at.
+__global__ void k_set_base(int64_t *base, int64_t v) { *base = v; }
+__global__ void k_advance_base(int64_t *base, int64_t k) { *base += k; }
 
 // Frozen-globals replay: nsteps SSA steps in one launch.
 template <int PRNG, bool DIGEST, class R>
@@ -783,6 +789,14 @@ struct gw_batch {
     int64_t trace_cap = 0;
     int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
     int32_t lanes = 4;               // lanes per CRU: 4 (quad) or 1 (FP32-local only)
+    // replayed graphs of step chunks (0 = plain launches)
+    int32_t graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int32_t graph_steps_built = 0, graph_lanes = 0, graph_shape = 0;
+    bool graph_digest = false;
+    KArgs graph_args{};
+    int64_t *d_step_base = nullptr;
     int64_t nblocks_shape[kNumShapes] = {0, 0, 0, 0};
     int64_t bc_off[kNumShapes] = {0, 0, 0, 0};   // offset of each shape's block -> cell map
     int64_t max_nblocks = 0;
@@ -850,6 +864,7 @@ bool shape_allowed(const gw_batch *B, int s, int lanes = 4)
 
 // The launch configurations the autotuner tries: (lanes per CRU, threads per block).
 constexpr int kNumConfigs = 6;
+constexpr int kAutotuneGraphSteps = 64;   // steps per captured graph the autotuner tries
 constexpr int kConfigLanes[kNumConfigs] = {4, 4, 4, 4, 1, 1};
 constexpr int kConfigThreads[kNumConfigs] = {64, 128, 256, 512, 64, 128};
 int config_shape(int c) { return kConfigLanes[c] == 4 ? shape_of_block(kConfigThreads[c]) : (kConfigThreads[c] == 64 ? 2 : 3); }
@@ -896,78 +911,138 @@ unsigned step_grid(const gw_batch *B, int64_t ssa_step, int32_t nclose)
     return static_cast<unsigned>(nclose + (ssa_step >= 0 ? B->nblocks : 0));
 }
 
+// A step launch: with timing events through the dispatch packet, without them
+// as a plain launch (the form stream capture records into a graph).
+template <class K>
+void dispatch(K kernel, unsigned grid, int threads, gw_batch *B, hipEvent_t e0, hipEvent_t e1, int64_t ssa_step,
+              int32_t nclose, const int64_t *base)
+{
+    if (e0)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose, base);
+    else
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, B->ctx->stream, B->a, ssa_step, nclose, base);
+}
+
 // One launch of k_step: SSA(ssa_step) (none if ssa_step < 0) beside the
 // closes of the previous step (when nclose > 0). When e0/e1 are given, the
 // dispatch itself records them (its start and end, no extra queue packets).
 template <int PRNG, int BLOCK>
-void launch_k_step_shape(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
+void launch_k_step_shape(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1, const int64_t *base)
 {
+    const unsigned grid = step_grid(B, ssa_step, nclose);
     if (B->precision == GW_PREC_FP32_LOCAL) {
-        const auto k = k_step<PRNG, false, float, BLOCK>;
-        const unsigned grid = step_grid(B, ssa_step, nclose);
-        hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+        dispatch(k_step<PRNG, false, float, BLOCK>, grid, BLOCK, B, e0, e1, ssa_step, nclose, base);
         return;
     }
     if constexpr (BLOCK == kBlock) {
         if (B->digest) {
-            const auto k = k_step<PRNG, true, double, BLOCK>;
-            const unsigned grid = step_grid(B, ssa_step, nclose);
-            hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+            dispatch(k_step<PRNG, true, double, BLOCK>, grid, BLOCK, B, e0, e1, ssa_step, nclose, base);
             return;
         }
     }
-    const auto k = k_step<PRNG, false, double, BLOCK>;
-    const unsigned grid = step_grid(B, ssa_step, nclose);
-    hipExtLaunchKernelGGL(k, dim3(grid), dim3(BLOCK), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
+    dispatch(k_step<PRNG, false, double, BLOCK>, grid, BLOCK, B, e0, e1, ssa_step, nclose, base);
 }
 
 // Shapes other than the default are built for xoshiro256** only (the engine
 // of the production path); shape_allowed() keeps the others off them.
 template <int PRNG>
-void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
+void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1, const int64_t *base)
 {
     if constexpr (PRNG == GW_PRNG_XOSHIRO256SS) {
         if (B->lanes == 1) {
-            if (B->shape == 2) {
-                const auto k = k_step_lane<PRNG, 64>;
-                const unsigned grid = step_grid(B, ssa_step, nclose);
-                hipExtLaunchKernelGGL(k, dim3(grid), dim3(64), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
-            } else {
-                const auto k = k_step_lane<PRNG, 128>;
-                const unsigned grid = step_grid(B, ssa_step, nclose);
-                hipExtLaunchKernelGGL(k, dim3(grid), dim3(128), 0, B->ctx->stream, e0, e1, 0u, B->a, ssa_step, nclose);
-            }
+            const unsigned grid = step_grid(B, ssa_step, nclose);
+            if (B->shape == 2)
+                dispatch(k_step_lane<PRNG, 64>, grid, 64, B, e0, e1, ssa_step, nclose, base);
+            else
+                dispatch(k_step_lane<PRNG, 128>, grid, 128, B, e0, e1, ssa_step, nclose, base);
             return;
         }
         switch (B->shape) {
-        case 0: launch_k_step_shape<PRNG, 64>(B, ssa_step, nclose, e0, e1); return;
-        case 1: launch_k_step_shape<PRNG, 128>(B, ssa_step, nclose, e0, e1); return;
-        case 3: launch_k_step_shape<PRNG, 512>(B, ssa_step, nclose, e0, e1); return;
+        case 0: launch_k_step_shape<PRNG, 64>(B, ssa_step, nclose, e0, e1, base); return;
+        case 1: launch_k_step_shape<PRNG, 128>(B, ssa_step, nclose, e0, e1, base); return;
+        case 3: launch_k_step_shape<PRNG, 512>(B, ssa_step, nclose, e0, e1, base); return;
         default: break;
         }
     }
-    launch_k_step_shape<PRNG, kBlock>(B, ssa_step, nclose, e0, e1);
+    launch_k_step_shape<PRNG, kBlock>(B, ssa_step, nclose, e0, e1, base);
 }
 
-void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1)
+void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1,
+                const int64_t *base = nullptr)
 {
     switch (B->prng) {
-    case 0: launch_k_step<0>(B, ssa_step, nclose, e0, e1); break;
-    case 1: launch_k_step<1>(B, ssa_step, nclose, e0, e1); break;
-    case 2: launch_k_step<2>(B, ssa_step, nclose, e0, e1); break;
-    case 3: launch_k_step<3>(B, ssa_step, nclose, e0, e1); break;
-    case 4: launch_k_step<4>(B, ssa_step, nclose, e0, e1); break;
-    default: launch_k_step<5>(B, ssa_step, nclose, e0, e1); break;
+    case 0: launch_k_step<0>(B, ssa_step, nclose, e0, e1, base); break;
+    case 1: launch_k_step<1>(B, ssa_step, nclose, e0, e1, base); break;
+    case 2: launch_k_step<2>(B, ssa_step, nclose, e0, e1, base); break;
+    case 3: launch_k_step<3>(B, ssa_step, nclose, e0, e1, base); break;
+    case 4: launch_k_step<4>(B, ssa_step, nclose, e0, e1, base); break;
+    default: launch_k_step<5>(B, ssa_step, nclose, e0, e1, base); break;
     }
+}
+
+// The replayable graph of graph_steps step launches (each SSA(base + k) beside
+// the close of the step before) and the launch that advances base. It is
+// captured again whenever the launch arguments or shape change.
+bool graph_current(const gw_batch *B)
+{
+    return B->graph_exec && B->graph_steps_built == B->graph_steps && B->graph_lanes == B->lanes &&
+           B->graph_shape == B->shape && B->graph_digest == B->digest &&
+           std::memcmp(&B->graph_args, &B->a, sizeof(KArgs)) == 0;
+}
+
+void graph_release(gw_batch *B)
+{
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->graph_exec = nullptr;
+    B->graph = nullptr;
+}
+
+hipError_t graph_build(gw_batch *B)
+{
+    graph_release(B);
+    hipStream_t st = B->ctx->stream;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < B->graph_steps; ++k) launch_one(B, k, B->ncells, nullptr, nullptr, B->d_step_base);
+    hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(1), 0, st, B->d_step_base, static_cast<int64_t>(B->graph_steps));
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return e;
+    B->graph = g;
+    e = hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        graph_release(B);
+        return e;
+    }
+    B->graph_steps_built = B->graph_steps;
+    B->graph_lanes = B->lanes;
+    B->graph_shape = B->shape;
+    B->graph_digest = B->digest;
+    B->graph_args = B->a;
+    return hipSuccess;
 }
 
 // Steps step0 .. step0+n-1: n launches, launch k running SSA(step0+k) beside
 // close(step0+k-1), then one launch with the last closes. Between calls every
 // step is closed, so the cell state read by the host is that of step0+n.
-void launch_steps(gw_batch *B, int64_t step0, int64_t n)
+// With graphs enabled, the middle launches are replays of the captured chunk.
+hipError_t launch_steps(gw_batch *B, int64_t step0, int64_t n)
 {
-    for (int64_t k = 0; k < n; ++k) launch_one(B, step0 + k, k == 0 ? 0 : B->ncells, nullptr, nullptr);
-    if (n > 0) launch_one(B, -1, B->ncells, nullptr, nullptr);
+    if (n <= 0) return hipSuccess;
+    launch_one(B, step0, 0, nullptr, nullptr);
+    int64_t k = 1;
+    if (B->graph_steps > 0 && n - 1 >= B->graph_steps) {
+        hipError_t e = hipSuccess;
+        if (!graph_current(B)) e = graph_build(B);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, B->ctx->stream, B->d_step_base, step0 + 1);
+        for (; k + B->graph_steps <= n; k += B->graph_steps)
+            if ((e = hipGraphLaunch(B->graph_exec, B->ctx->stream)) != hipSuccess) return e;
+    }
+    for (; k < n; ++k) launch_one(B, step0 + k, B->ncells, nullptr, nullptr);
+    launch_one(B, -1, B->ncells, nullptr, nullptr);
+    return hipGetLastError();
 }
 
 // launch_steps with every step's dispatch timed through a ring of event pairs:
@@ -1176,8 +1251,9 @@ int gw_batch_destroy(gw_batch *B)
     live.erase(std::remove(live.begin(), live.end(), B), live.end());
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
+    graph_release(B);
     void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
-                    B->a.trace, B->a.iters_total, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
+                    B->a.trace, B->a.iters_total, B->d_step_base, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : B->istim_dev)
@@ -1273,7 +1349,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
         (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
         (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->max_nblocks, err, errlen)) ||
-        (rc = dmalloc(&a.iters_total, 1, err, errlen)) ||
+        (rc = dmalloc(&a.iters_total, 1, err, errlen)) || (rc = dmalloc(&B->d_step_base, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
         (rc = dmalloc(&B->d_block_cell, B->block_cell_h.size(), err, errlen))) {
         gw_batch_destroy(B);
@@ -1478,8 +1554,7 @@ int gw_batch_step(gw_batch *B, int64_t nsteps, char *err, size_t errlen)
     const int64_t chunk = 4096;
     for (int64_t done = 0; done < nsteps;) {
         const int64_t n = std::min(chunk, nsteps - done);
-        launch_steps(B, B->steps_done, n);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(launch_steps(B, B->steps_done, n));
         done += n;
         B->steps_done += n;
         rc = check_status(B, err, errlen);
@@ -1637,8 +1712,8 @@ int gw_batch_get_launch(gw_batch *B, int32_t *lanes_per_cru, int32_t *block)
     return GW_OK;
 }
 
-int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out, double *ms_per_step,
-                      char *err, size_t errlen)
+int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out, int32_t *graph_out,
+                      double *ms_per_step, char *err, size_t errlen)
 {
     if (!B || trial_steps <= 0) {
         set_err(err, errlen, "gw_batch_autotune: bad arguments");
@@ -1689,11 +1764,10 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         if (ms_per_step) ms_per_step[c] = -1.0;
         if (!shape_allowed(B, config_shape(c), kConfigLanes[c])) continue;
         set_shape(B, config_shape(c), kConfigLanes[c]);
-        launch_steps(B, B->steps_done, 2);
+        if ((he = launch_steps(B, B->steps_done, 2)) != hipSuccess) break;
         B->steps_done += 2;
         if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
-        launch_steps(B, B->steps_done, trial_steps);
-        if ((he = hipGetLastError()) != hipSuccess) break;
+        if ((he = launch_steps(B, B->steps_done, trial_steps)) != hipSuccess) break;
         if ((he = hipEventRecord(e1, st)) != hipSuccess) break;
         if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
         float f = 0.f;
@@ -1708,6 +1782,29 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         he = copy_all(false);
         B->steps_done = steps0;
     }
+    // the fastest shape once more with its steps replayed as captured graphs
+    const int32_t graph0 = B->graph_steps;
+    int32_t best_graph = 0;
+    if (ms_per_step) ms_per_step[kNumConfigs] = -1.0;
+    if (he == hipSuccess && trial_steps > kAutotuneGraphSteps) {
+        set_shape(B, best_shape, best_lanes);
+        B->graph_steps = kAutotuneGraphSteps;
+        if ((he = launch_steps(B, B->steps_done, 2)) == hipSuccess) {
+            B->steps_done += 2;
+            if ((he = hipEventRecord(e0, st)) == hipSuccess && (he = launch_steps(B, B->steps_done, trial_steps)) == hipSuccess &&
+                (he = hipEventRecord(e1, st)) == hipSuccess && (he = hipEventSynchronize(e1)) == hipSuccess) {
+                float f = 0.f;
+                if ((he = hipEventElapsedTime(&f, e0, e1)) == hipSuccess) {
+                    const double ms = f / static_cast<double>(trial_steps);
+                    if (ms_per_step) ms_per_step[kNumConfigs] = ms;
+                    if (ms < best_ms) best_graph = kAutotuneGraphSteps;
+                }
+            }
+        }
+        if (he == hipSuccess) he = copy_all(false);
+        B->steps_done = steps0;
+    }
+    B->graph_steps = (he == hipSuccess) ? best_graph : graph0;
     if (he == hipSuccess) he = copy_all(false);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     B->steps_done = steps0;
@@ -1720,6 +1817,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         return GW_E_HIP;
     }
     gw_batch_get_launch(B, lanes_out, block_out);
+    if (graph_out) *graph_out = B->graph_steps;
     return check_status(B, err, errlen);
 }
 
@@ -1741,7 +1839,7 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
         rc = launch_steps_timed(B, B->steps_done, nsteps, ssa_ms, err, errlen);
         if (rc) return rc;
     } else {
-        launch_steps(B, B->steps_done, nsteps);
+        HIP_TRY(launch_steps(B, B->steps_done, nsteps));
     }
     HIP_TRY(hipGetLastError());
     B->steps_done += nsteps;
@@ -1754,6 +1852,16 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
     *ms = f;
     if (launches) *launches = nsteps;
     return check_status(B, err, errlen);
+}
+
+int gw_batch_set_graph(gw_batch *B, int32_t steps_per_graph, char *err, size_t errlen)
+{
+    if (!B || steps_per_graph < 0 || steps_per_graph > 4096) {
+        set_err(err, errlen, "gw_batch_set_graph: steps_per_graph must be in 0..4096");
+        return GW_E_INVALID_ARG;
+    }
+    B->graph_steps = steps_per_graph;
+    return GW_OK;
 }
 
 // Internal switches used by gw_run and the replay tests.

@@ -253,6 +253,13 @@ int gw_batch_get_globals(gw_batch *b, gw_globals_t *g_out /* [ncells] */,
 /* Per-step table stimulus for one cell (GW_PROTO_TABLE); `istim` has n entries, copied to the device. */
 int gw_batch_set_stimulus_table(gw_batch *b, int32_t cell, const double *istim, int64_t n,
                                 char *err, size_t errlen);
+/*
+ * Replay steps as captured HIP graphs of steps_per_graph launches (0 = plain
+ * launches, the default). The graph is captured on first use and again after
+ * any change of launch shape or mode; results are identical either way.
+ * No reference counterpart.
+ */
+int gw_batch_set_graph(gw_batch *b, int32_t steps_per_graph, char *err, size_t errlen);
 /* Advance every cell by nsteps Euler steps. */
 int gw_batch_step(gw_batch *b, int64_t nsteps, char *err, size_t errlen);
 /* Simulated time of the batch (accumulated t += dt, as src/pyGW.hpp:116). */
@@ -293,12 +300,15 @@ int gw_batch_get_launch(gw_batch *b, int32_t *lanes_per_cru, int32_t *block);
 /*
  * Autotuner: times trial_steps Euler steps of the batch in every available
  * launch shape, each trial starting from the same state (the batch state is
- * saved before and restored after), and keeps the fastest. ms_per_step[6]
- * (may be NULL) receives the time per step of the shapes 4x64, 4x128, 4x256,
- * 4x512, 1x64 and 1x128 (lanes per CRU x threads; -1 where not available).
+ * saved before and restored after), keeps the fastest, then times that shape
+ * with its steps replayed as captured graphs of 64 steps and keeps graphs if
+ * they are faster (*graph_out: 64 or 0). ms_per_step[7] (may be NULL)
+ * receives the time per step of the shapes 4x64, 4x128, 4x256, 4x512, 1x64
+ * and 1x128 (lanes per CRU x threads; -1 where not available) and of the
+ * graph trial.
  */
 int gw_batch_autotune(gw_batch *b, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out,
-                      double *ms_per_step, char *err, size_t errlen);
+                      int32_t *graph_out, double *ms_per_step, char *err, size_t errlen);
 /*
  * Time nsteps Euler steps with HIP events on the batch's stream: *ms is the
  * wall time of the whole sequence, *launches the number of SSA dispatches.

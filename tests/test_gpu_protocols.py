@@ -205,3 +205,36 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
     assert n_a == n_b == 600 and t_a == t_b
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision,launch,digest", [(0, (4, 256), True), (0, (4, 64), False), (1, (1, 128), False)],
+                         ids=["fp64-digest", "fp64-4x64", "fp32-1x128"])
+def test_graph_replays_match_plain_launches(gwcapi, gpu_ctx, precision, launch, digest):
+    """Steps replayed as captured graphs give the same bits as plain launches,
+    over calls of several lengths with the host reading the state between them."""
+    say(f"graphs {precision} {launch}")
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    cells = [gwcapi.make_cell(n, s, None, pace) for n, s in ((300, 5), (1000, 6))]
+    runs = []
+    for graph in (0, 16):
+        b = gwcapi.Batch(gpu_ctx, cells, precision=precision)
+        b.set_mode(frozen=False, digest=digest)
+        b.set_launch(*launch)
+        b.set_graph(graph)
+        runs.append(b)
+    for chunk in (1, 40, 17, 100, 3, 250):
+        for b in runs:
+            b.step(chunk)
+        ga, sa = _snapshot(runs[0], len(cells))
+        gb, sb = _snapshot(runs[1], len(cells))
+        for x, y in zip(ga, gb):
+            assert np.array_equal(x, y), chunk
+        for x, y in zip(sa, sb):
+            assert_states_equal(x, y)
+        if digest:
+            for c in range(len(cells)):
+                da, db = runs[0].digest(c), runs[1].digest(c)
+                assert np.array_equal(da[0], db[0]) and np.array_equal(da[1], db[1])
+        assert runs[0].time() == runs[1].time()
+    for b in runs:
+        b.close()
