@@ -865,6 +865,7 @@ bool shape_allowed(const gw_batch *B, int s, int lanes = 4)
 // The launch configurations the autotuner tries: (lanes per CRU, threads per block).
 constexpr int kNumConfigs = 6;
 constexpr int kAutotuneGraphSteps = 64;   // steps per captured graph the autotuner tries
+constexpr int kAutotuneWarm = kAutotuneGraphSteps + 2;   // untimed steps before every trial
 constexpr int kConfigLanes[kNumConfigs] = {4, 4, 4, 4, 1, 1};
 constexpr int kConfigThreads[kNumConfigs] = {64, 128, 256, 512, 64, 128};
 int config_shape(int c) { return kConfigLanes[c] == 4 ? shape_of_block(kConfigThreads[c]) : (kConfigThreads[c] == 64 ? 2 : 3); }
@@ -1764,8 +1765,8 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         if (ms_per_step) ms_per_step[c] = -1.0;
         if (!shape_allowed(B, config_shape(c), kConfigLanes[c])) continue;
         set_shape(B, config_shape(c), kConfigLanes[c]);
-        if ((he = launch_steps(B, B->steps_done, 2)) != hipSuccess) break;
-        B->steps_done += 2;
+        if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) != hipSuccess) break;
+        B->steps_done += kAutotuneWarm;
         if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
         if ((he = launch_steps(B, B->steps_done, trial_steps)) != hipSuccess) break;
         if ((he = hipEventRecord(e1, st)) != hipSuccess) break;
@@ -1789,8 +1790,9 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
     if (he == hipSuccess && trial_steps > kAutotuneGraphSteps) {
         set_shape(B, best_shape, best_lanes);
         B->graph_steps = kAutotuneGraphSteps;
-        if ((he = launch_steps(B, B->steps_done, 2)) == hipSuccess) {
-            B->steps_done += 2;
+        // the warm-up captures and replays the graph once, outside the timed window
+        if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) == hipSuccess) {
+            B->steps_done += kAutotuneWarm;
             if ((he = hipEventRecord(e0, st)) == hipSuccess && (he = launch_steps(B, B->steps_done, trial_steps)) == hipSuccess &&
                 (he = hipEventRecord(e1, st)) == hipSuccess && (he = hipEventSynchronize(e1)) == hipSuccess) {
                 float f = 0.f;

@@ -238,3 +238,54 @@ def test_graph_replays_match_plain_launches(gwcapi, gpu_ctx, precision, launch, 
         assert runs[0].time() == runs[1].time()
     for b in runs:
         b.close()
+
+
+def _apd90_from_trace(V, t, v0):
+    """APD90 of every beat from a sampled V trace, by the definition the device
+    tracker uses: upstroke at the upward -40 mV crossing, end at the downward
+    crossing of V_rest + 0.1 (V_peak - V_rest), both interpolated linearly."""
+    out = []
+    t_up, v_rest, v_peak, v_old, t_old = -1.0, v0, v0, v0, 0.0
+    for v, tt in zip(V, t):
+        if t_up < 0:
+            v_rest = min(v_rest, v)
+            if v_old < -40.0 <= v:
+                t_up = t_old + (tt - t_old) * (-40.0 - v_old) / (v - v_old)
+                v_peak = v
+        else:
+            v_peak = max(v_peak, v)
+            thr = v_rest + 0.1 * (v_peak - v_rest)
+            if v_old > thr and v <= thr and v < v_peak - 10.0:
+                out.append(t_old + (tt - t_old) * (v_old - thr) / (v_old - v) - t_up)
+                t_up, v_rest = -1.0, v
+        v_old, t_old = v, tt
+    return out
+
+
+def test_on_device_apd_and_alternans_match_the_trace(gwcapi, gpu_ctx):
+    """The per-cell observables (APD90 of the last two beats, mean beat-to-beat
+    APD90 difference, beats, peaks) agree with the same quantities computed on
+    the host from a V trace taken every second step, over three beats at 2 Hz."""
+    ncell, n, period, beats, every = 4, 500, 500.0, 3, 2
+    steps = int(beats * period * 1000)
+    say(f"observables {ncell} cells x {n} CRUs, {steps} steps")
+    pace = {"kind": gwcapi.PROTO_PACE, "period": period, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, 70 + c, None, pace) for c in range(ncell)])
+    b.set_mode(frozen=False, digest=False)
+    b.enable_trace(every, steps // every + 8)
+    b.step(steps)
+    tr = b.get_trace()
+    obs = b.obs()
+    b.close()
+    assert tr.shape[0] == steps // every
+    t = np.arange(1, tr.shape[0] + 1) * every * 1e-3
+    for c in range(ncell):
+        apds = _apd90_from_trace(tr[:, c, 0], t, gwcapi.default_globals().V)
+        o = obs[c]
+        say(f"  cell {c}: device beats {o['beats_done']:.0f} APD90 {o['apd90_prev']:.3f}, {o['apd90_last']:.3f}; "
+            f"trace {[round(x, 3) for x in apds]}")
+        assert len(apds) >= 2 and o["beats_done"] == len(apds)
+        assert abs(o["apd90_last"] - apds[-1]) < 0.02 and abs(o["apd90_prev"] - apds[-2]) < 0.02
+        assert abs(o["alt_sum"] / o["alt_count"] - float(np.mean(np.abs(np.diff(apds))))) < 0.02
+        assert o["peak_Cai"] >= tr[:, c, 1].max() and o["peak_ICaL_neg"] <= tr[:, c, 2].min()
+        assert abs(o["sum_open_RyR_frac"] / o["n_steps"] - tr[:, c, 3].mean()) < 2e-3
