@@ -136,11 +136,13 @@ def lib() -> C.CDLL:
         "gw_batch_get_event_digest": (C.c_int, [P, i32, P, P] + E),
         "gw_batch_set_mode": (C.c_int, [P, i32, i32]),
         "gw_batch_get_iterations": (C.c_int, [P, C.POINTER(u64)]),
-        "gw_batch_set_launch": (C.c_int, [P, i32, i32] + E),
+        "gw_batch_set_launch": (C.c_int, [P, i32, i32, i32] + E),
+        "gw_launch_config": (C.c_int, [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
-        "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
+        "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "gw_batch_set_graph": (C.c_int, [P, i32] + E),
-        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)] + E),
+        "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                        C.POINTER(f64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
     }
     for name, (res, args) in sig.items():
@@ -196,6 +198,22 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def launch_configs() -> list[tuple[int, int, int]]:
+    """The launch configurations (lanes per CRU, threads per block, CRUs per
+    lane) in gw_launch_config order, the order of the autotuner's timings."""
+    L, out, k = lib(), [], 0
+    lanes, blk, cpl = i32(), i32(), i32()
+    while L.gw_launch_config(k, C.byref(lanes), C.byref(blk), C.byref(cpl)) == 0:
+        out.append((lanes.value, blk.value, cpl.value))
+        k += 1
+    return out
+
+
+def launch_name(lanes: int, block: int, crus_per_lane: int = 1) -> str:
+    """'4x256' (lanes per CRU x threads), '1x64x4' with CRUs per lane above 1."""
+    return f"{lanes}x{block}" + (f"x{crus_per_lane}" if crus_per_lane > 1 else "")
 
 
 def make_cell(n_cru: int, seed: int = 42, params: Params | None = None, protocol: dict | None = None) -> CellDesc:
@@ -307,32 +325,35 @@ class Batch:
         self.L.gw_batch_get_iterations(self.h, C.byref(v))
         return v.value
 
-    SHAPES = ((4, 64), (4, 128), (4, 256), (4, 512), (1, 64), (1, 128))
-
-    def set_launch(self, lanes_per_cru: int = 0, block: int = 0):
-        _call(self.L.gw_batch_set_launch, self.h, lanes_per_cru, block)
+    def set_launch(self, lanes_per_cru: int = 0, block: int = 0, crus_per_lane: int = 0):
+        """gw_batch_set_launch: lanes per CRU (4 or 1), threads per block and,
+        with one lane per CRU, CRUs per lane (1, 2, 4 or 8); 0 = default."""
+        _call(self.L.gw_batch_set_launch, self.h, lanes_per_cru, block, crus_per_lane)
 
     def set_graph(self, steps_per_graph: int):
         """Replay step launches as captured HIP graphs of this many steps (0: plain launches)."""
         _call(self.L.gw_batch_set_graph, self.h, int(steps_per_graph))
 
-    def launch(self) -> tuple[int, int]:
-        """(lanes per CRU, threads per block) of the current SSA launch shape."""
-        lanes, blk = i32(), i32()
-        self.L.gw_batch_get_launch(self.h, C.byref(lanes), C.byref(blk))
-        return lanes.value, blk.value
+    def launch(self) -> tuple[int, int, int]:
+        """(lanes per CRU, threads per block, CRUs per lane) of the current SSA launch shape."""
+        lanes, blk, cpl = i32(), i32(), i32()
+        self.L.gw_batch_get_launch(self.h, C.byref(lanes), C.byref(blk), C.byref(cpl))
+        return lanes.value, blk.value, cpl.value
 
-    def autotune(self, trial_steps: int = 200) -> tuple[tuple[int, int], dict]:
+    def autotune(self, trial_steps: int = 200) -> tuple[tuple[int, int, int], dict]:
         """gw_batch_autotune: pick the fastest SSA launch shape, with or without
         graph replay, on this batch (its state is left unchanged). Returns the
-        shape and the trial times per step; `graph` is the chosen graph size."""
-        lanes, blk, graph = i32(), i32(), i32()
-        ms = (f64 * 7)()
-        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(lanes), C.byref(blk), C.byref(graph), ms)
-        trials = {f"{l}x{b}": (ms[k] if ms[k] >= 0 else None) for k, (l, b) in enumerate(self.SHAPES)}
-        trials["graph64"] = ms[6] if ms[6] >= 0 else None
+        shape and the trial times per step keyed by launch_name(); `graph_steps`
+        is the chosen graph size."""
+        lanes, blk, cpl, graph = i32(), i32(), i32(), i32()
+        shapes = launch_configs()
+        ms = (f64 * (len(shapes) + 1))()
+        _call(self.L.gw_batch_autotune, self.h, int(trial_steps), C.byref(lanes), C.byref(blk), C.byref(cpl),
+              C.byref(graph), ms)
+        trials = {launch_name(*sh): (ms[k] if ms[k] >= 0 else None) for k, sh in enumerate(shapes)}
+        trials["graph64"] = ms[len(shapes)] if ms[len(shapes)] >= 0 else None
         self.graph_steps = graph.value
-        return (lanes.value, blk.value), trials
+        return (lanes.value, blk.value, cpl.value), trials
 
     def time_steps(self, nsteps: int, per_kernel: bool = False):
         """(total ms, SSA-kernel ms or None, SSA launches) of nsteps Euler steps."""

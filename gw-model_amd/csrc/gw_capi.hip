@@ -38,13 +38,13 @@ namespace {
 // ===========================================================================
 // device helpers
 // ===========================================================================
-__device__ double protocol_istim(const DevCell &C, const CellDyn &d)
+__device__ double protocol_istim(const DevCell &C, double t, int64_t step)
 {
     switch (C.proto.kind) {
     case GW_PROTO_TABLE:
-        return (C.istim && d.step < C.istim_len) ? C.istim[d.step] : 0.0;
+        return (C.istim && step < C.istim_len) ? C.istim[step] : 0.0;
     case GW_PROTO_PACE: {
-        const double x = d.t - C.proto.start;
+        const double x = t - C.proto.start;
         if (x < 0.0) return 0.0;
         const double ph = (C.proto.period > 0.0 && C.proto.period < 1e300) ? fmod(x, C.proto.period) : x;
         return ph < C.proto.duration ? C.proto.amplitude : 0.0;
@@ -103,15 +103,15 @@ __device__ void obs_update(gw_cell_obs_t &o, double V_old, double V, double t_ol
 // globals and constants; the results are written back to the cell's state once.
 __device__ void prepare_step(const DevCell &C, CellDyn &d, const FluxSums &S, double dt, Currents &cur, int kslot)
 {
-    const gw_params_t p = C.p;
-    const CellConsts c = C.c;
+    const gw_params_t &p = C.p;
+    const CellConsts &c = C.c;
     gw_globals_t g = d.g;
     gw_globals_t dg;
-    const double Istim = protocol_istim(C, d);
+    const double Istim = protocol_istim(C, d.t, d.step);
     if (d.clamp) g.V = protocol_vclamp(C, d.t);
     const double VF_RT = g.V * c.F_RT;
     const double expm = gw::exp(-VF_RT);
-    const int rc = global_derivatives(p, c, g, VF_RT, expm, dt, Istim, S, dg, cur);
+    const int rc = global_derivatives(p, c, g, VF_RT, dt, Istim, S, dg, cur);
     if (rc != GW_OK && d.status == 0) d.status = rc;
     d.Istim = Istim;
     d.g = g;
@@ -451,13 +451,32 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     }
 }
 
+// LDS of a close block. The working set of the close reuses the slots of the
+// partial-sum reduction once its total has been read.
+struct CloseLds {
+    double g[36];            // the globals the derivatives start from (gw_globals_t order)
+    double x[GD_N];          // arguments, then values, of the batched transcendentals
+    double py[GD_N - GD_POW0];
+    double sx[SC_N];         // arguments, then values, of the next step's constants
+    double lookahead[3];     // V, Cai, CaNSR of the next step
+};
+union CloseShared {
+    Partial red[kWave];
+    CloseLds L;
+};
+static_assert(sizeof(gw_globals_t) == 36 * sizeof(double), "globals are 36 doubles");
+
 // Close step n = D.step of one cell: the exact sum of the cell's block
 // partials of step n, globals += dGlobals (GW.tpp:252-271), t += dt, the
 // derivatives of step n+1, observables and trace, and the SSA constants of
 // step n+2 (those of step n+1 were formed one close earlier and are in use by
-// the SSA blocks running beside this one).
+// the SSA blocks running beside this one). The step's 42 exponentials,
+// logarithms and powers that depend on the state alone are evaluated one per
+// lane, and the 5 exponentials of the next step's constants likewise; the
+// rest is the serial arithmetic of prepare_step and lookahead_consts. Every
+// thread of the block takes part (the block synchronises).
 template <int NS>
-__device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *red)
+__device__ __forceinline__ void close_cell(const KArgs &a, int cell, CloseShared &sh)
 {
     const DevCell &C = a.cells[cell];
     CellDyn &D = a.dyn[cell];
@@ -467,36 +486,90 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, Partial *re
     partial_zero(mine);
     if (tid < NS)
         for (int q = tid; q < C.nblk[a.shape]; q += NS) partial_add(mine, a.partials[ks * a.nblocks + C.blk0[a.shape] + q]);
-    reduce_slots<NS>(red, mine, tid, tid < NS);
-    if (threadIdx.x != 0) return;
-    const Partial &tot = red[0];
-    if (a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
-    if (D.status != 0) return;
+    reduce_slots<NS>(sh.red, mine, tid, tid < NS);
+    const Partial tot = sh.red[0];
+    const int status = D.status;
+    const bool clamp = D.clamp != 0;
+    const double t_old = D.t;
+    const double V_old = D.g.V;
+    const int64_t step_new = D.step + 1;
+    __syncthreads();   // the reduction slots are reused below
+    if (tid == 0 && a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
+    if (status != 0) return;
     if (tot.err) {
-        D.status = GW_E_RANGE;
+        if (tid == 0) D.status = GW_E_RANGE;
         return;
     }
+    CloseLds &L = sh.L;
     const FluxSums S{fx_to_double(tot.jlcc), fx_to_double(tot.jxfer), fx_to_double(tot.jtr), static_cast<double>(tot.clch)};
     const double jrel = fx_to_double(tot.jrel);
+    const double t_new = t_old + a.dt;
+    // globals_apply (GW.tpp:252-271), one global per thread; V is held under a
+    // clamp and then set to the clamp potential of the new time (prepare_step)
+    const double *gD = reinterpret_cast<const double *>(&D.g);
+    const double *dgD = reinterpret_cast<const double *>(&D.dg);
+    if (tid < 36) {
+        double v = gD[tid];
+        if (!a.frozen && !(tid == 0 && clamp)) v += dgD[tid];
+        if (tid == 0 && clamp) v = protocol_vclamp(C, t_new);
+        L.g[tid] = v;
+    }
+    __syncthreads();
+    const gw_globals_t &g = *reinterpret_cast<const gw_globals_t *>(L.g);
+    const double VF_RT = g.V * C.c.F_RT;
+    if (tid == 0) gd_args(C.p, g, VF_RT, L.x, L.py);
+    __syncthreads();
+    if (tid < GD_N) {
+        const double v = gd_eval(tid, L.x[tid], tid >= GD_POW0 ? L.py[tid - GD_POW0] : 0.0);
+        L.x[tid] = v;   // each thread reads and then overwrites its own slot
+    }
+    __syncthreads();
+    Currents cur;
+    gw_globals_t dg;
+    if (tid == 0) {
+        const double Istim = protocol_istim(C, t_new, step_new);
+        const int rc = gd_finish(C.p, C.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
+        if (rc != GW_OK) D.status = rc;
+        D.Istim = Istim;
+        D.dg = dg;
+        // lookahead_consts: the potential and concentrations the step after starts from
+        double V = g.V, Cai = g.Cai, CaNSR = g.CaNSR;
+        if (!a.frozen) {
+            if (!clamp) V += dg.V;
+            Cai += dg.Cai;
+            CaNSR += dg.CaNSR;
+        }
+        if (clamp) V = protocol_vclamp(C, t_new + a.dt);
+        sc_args(V, C.c.F_RT, L.sx);
+        L.lookahead[0] = V;
+        L.lookahead[1] = Cai;
+        L.lookahead[2] = CaNSR;
+    }
+    __syncthreads();
+    if (tid < SC_N) {
+        const double v = gw::exp(L.sx[tid]);
+        L.sx[tid] = v;
+    }
+    __syncthreads();
+    if (tid < 36) reinterpret_cast<double *>(&D.g)[tid] = L.g[tid];
+    if (tid != 0) return;
+    const int slot = static_cast<int>((step_new + 1) & 1);
+    step_consts_x(D.k[slot], C.p, C.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
+                  static_cast<uint32_t>((step_new + 1) & 0xffff));
+    step_consts_f32(D.kf[slot], D.k[slot]);
     D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
     D.sums[4] = static_cast<double>(tot.open); D.sums[5] = jrel;
-    const double V_old = D.g.V;
-    const double t_old = D.t;
-    if (!a.frozen) globals_apply(D.g, D.dg, D.clamp != 0);
-    D.t += a.dt;
-    D.step += 1;
-    Currents cur;
-    prepare_step(C, D, S, a.dt, cur, -1);
-    lookahead_consts(C, D, a.dt, a.frozen != 0);
-    if (!(D.g.V == D.g.V) || !(D.g.Cai == D.g.Cai)) {
+    D.t = t_new;
+    D.step = step_new;
+    if (!(g.V == g.V) || !(g.Cai == g.Cai)) {
         if (D.status == 0) D.status = GW_E_NAN;
     }
     const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
-    obs_update(D.obs, V_old, D.g.V, t_old, a.dt, D.g.Cai, cur.ICaL, open_frac, jrel * C.c.CRU_factor);
-    if (a.trace_every > 0 && (D.step % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
+    obs_update(D.obs, V_old, g.V, t_old, a.dt, g.Cai, cur.ICaL, open_frac, jrel * C.c.CRU_factor);
+    if (a.trace_every > 0 && (step_new % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
         double *row = a.trace + (D.trace_rows * a.ncells + cell) * 4;
-        row[0] = D.g.V;
-        row[1] = D.g.Cai;
+        row[0] = g.V;
+        row[1] = g.Cai;
         row[2] = cur.ICaL;
         row[3] = open_frac;
         D.trace_rows += 1;
@@ -519,20 +592,18 @@ __global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64
                                                                    const int64_t *base)
 {
     __shared__ LccTableT<R> T;
-    __shared__ Partial red[kWave];
+    __shared__ CloseShared csh;
     __shared__ BlockAcc acc;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kWave>(a, bid, red);
+        close_cell<kWave>(a, bid, csh);
         return;
     }
     ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, 1, T, acc);
 }
 
-// The SSA of the 64 or 128 CRUs of block b with one lane per CRU (FP32-local
-// variant). Per-wave sums of the CRU outputs are formed in FP64 by a fixed
-// xor butterfly (the same value in every lane), then added exactly across
-// waves and blocks as in ssa_block.
+// Sum over the 64 lanes of a wave by a fixed xor butterfly (the same value,
+// bit for bit, in every lane).
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -546,55 +617,79 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-template <int PRNG, int BLOCK>
-__device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t step0, LccTableF &T, BlockAcc &acc)
+__device__ __forceinline__ void lane_load(const KArgs &a, int64_t slot, LaneState &S)
 {
-    constexpr int kShape = (BLOCK == 64) ? 2 : 3;   // the block -> cell map of 64 or 128 CRUs per block
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
-    const int ks = static_cast<int>(step0 & 1);
-    const DevCell &C = a.cells[cell];
-    const CellDyn &D = a.dyn[cell];
-    const SsaParamsF &sp = C.spf;
-    const StepConstsF &kk = D.kf[ks];
-    load_lcc_table(kk, T);
-    acc_zero(acc);
-    __syncthreads();
-    const int64_t i = (b - C.blk0[kShape]) * BLOCK + tid;
-    const bool active = (i < C.n_cru) && (D.status == 0);
-    LaneOut o{};
-    if (active) {
-        const int64_t slot = C.slot0 + i;
-        double ca[4];
-        float jl[4];
-        uint64_t w[4];
-        CruQ q;
-        const double2 *pc = reinterpret_cast<const double2 *>(a.CaSS + slot * 4);
-        const double2 c01 = pc[0], c23 = pc[1];
-        ca[0] = c01.x; ca[1] = c01.y; ca[2] = c23.x; ca[3] = c23.y;
-        const float4 j4 = *reinterpret_cast<const float4 *>(a.JLCCf + slot * 4);
-        jl[0] = j4.x; jl[1] = j4.y; jl[2] = j4.z; jl[3] = j4.w;
-        const ulonglong2 *pw = reinterpret_cast<const ulonglong2 *>(a.sub + slot * 4);
-        const ulonglong2 w01 = pw[0], w23 = pw[1];
-        w[0] = w01.x; w[1] = w01.y; w[2] = w23.x; w[3] = w23.y;
-        const ulonglong2 *pr = reinterpret_cast<const ulonglong2 *>(a.rng + slot * 4);
-        const ulonglong2 r01 = pr[0], r23 = pr[1];
-        q.r0 = r01.x; q.r1 = r01.y; q.r2 = r23.x; q.r3 = r23.y;
-        q.jsr = a.CaJSR[slot];
-        o = ssa_step_lane<PRNG>(ca, jl, w, q, sp, kk, T, static_cast<float>(a.dt));
-        double2 *qc = reinterpret_cast<double2 *>(a.CaSS + slot * 4);
-        qc[0] = make_double2(ca[0], ca[1]);
-        qc[1] = make_double2(ca[2], ca[3]);
-        *reinterpret_cast<float4 *>(a.JLCCf + slot * 4) = make_float4(jl[0], jl[1], jl[2], jl[3]);
-        ulonglong2 *qw = reinterpret_cast<ulonglong2 *>(a.sub + slot * 4);
-        qw[0] = make_ulonglong2(w[0], w[1]);
-        qw[1] = make_ulonglong2(w[2], w[3]);
-        ulonglong2 *qr = reinterpret_cast<ulonglong2 *>(a.rng + slot * 4);
-        qr[0] = make_ulonglong2(q.r0, q.r1);
-        qr[1] = make_ulonglong2(q.r2, q.r3);
-        a.CaJSR[slot] = q.jsr;
+    const double2 *pc = reinterpret_cast<const double2 *>(a.CaSS + slot * 4);
+    const double2 c01 = pc[0], c23 = pc[1];
+    S.ca[0] = c01.x; S.ca[1] = c01.y; S.ca[2] = c23.x; S.ca[3] = c23.y;
+    const float4 j4 = *reinterpret_cast<const float4 *>(a.JLCCf + slot * 4);
+    S.jl[0] = j4.x; S.jl[1] = j4.y; S.jl[2] = j4.z; S.jl[3] = j4.w;
+    const ulonglong2 *pw = reinterpret_cast<const ulonglong2 *>(a.sub + slot * 4);
+    const ulonglong2 w01 = pw[0], w23 = pw[1];
+    S.w[0] = w01.x; S.w[1] = w01.y; S.w[2] = w23.x; S.w[3] = w23.y;
+    const ulonglong2 *pr = reinterpret_cast<const ulonglong2 *>(a.rng + slot * 4);
+    const ulonglong2 r01 = pr[0], r23 = pr[1];
+    S.q.r0 = r01.x; S.q.r1 = r01.y; S.q.r2 = r23.x; S.q.r3 = r23.y;
+    S.q.jsr = a.CaJSR[slot];
+}
+
+__device__ __forceinline__ void lane_store(const KArgs &a, int64_t slot, const LaneState &S)
+{
+    double2 *qc = reinterpret_cast<double2 *>(a.CaSS + slot * 4);
+    qc[0] = make_double2(S.ca[0], S.ca[1]);
+    qc[1] = make_double2(S.ca[2], S.ca[3]);
+    *reinterpret_cast<float4 *>(a.JLCCf + slot * 4) = make_float4(S.jl[0], S.jl[1], S.jl[2], S.jl[3]);
+    ulonglong2 *qw = reinterpret_cast<ulonglong2 *>(a.sub + slot * 4);
+    qw[0] = make_ulonglong2(S.w[0], S.w[1]);
+    qw[1] = make_ulonglong2(S.w[2], S.w[3]);
+    ulonglong2 *qr = reinterpret_cast<ulonglong2 *>(a.rng + slot * 4);
+    qr[0] = make_ulonglong2(S.q.r0, S.q.r1);
+    qr[1] = make_ulonglong2(S.q.r2, S.q.r3);
+    a.CaJSR[slot] = S.q.jsr;
+}
+
+// Running sums of the outputs of the CRUs each lane finishes, in finishing
+// order, one LDS slot per thread (kept out of the registers the SSA needs).
+template <int BLOCK>
+struct LaneSums {
+    double Jxfer[BLOCK], Jtr[BLOCK], JLCC[BLOCK], Jrel[BLOCK];
+    int open[BLOCK], clch[BLOCK], iters[BLOCK], err[BLOCK];
+    __device__ __forceinline__ void zero(int t)
+    {
+        Jxfer[t] = Jtr[t] = JLCC[t] = Jrel[t] = 0.0;
+        open[t] = clch[t] = iters[t] = err[t] = 0;
     }
+    __device__ __forceinline__ void add(int t, const LaneOut &o)
+    {
+        Jxfer[t] += o.Jxfer;
+        Jtr[t] += o.Jtr;
+        JLCC[t] += o.JLCC;
+        Jrel[t] += o.Jrel;
+        open[t] += o.open;
+        clch[t] += o.clch;
+        iters[t] += static_cast<int>(o.iters);
+        err[t] |= o.err;
+    }
+    __device__ __forceinline__ LaneOut get(int t) const
+    {
+        LaneOut o;
+        o.Jxfer = Jxfer[t];
+        o.Jtr = Jtr[t];
+        o.JLCC = JLCC[t];
+        o.Jrel = Jrel[t];
+        o.open = open[t];
+        o.clch = clch[t];
+        o.iters = static_cast<uint32_t>(iters[t]);
+        o.err = err[t];
+        return o;
+    }
+};
+
+// Per-wave sums of the lanes' output sums (a fixed FP64 xor butterfly, the
+// same in every lane), added exactly into the block accumulators by lanes 0..3.
+__device__ __forceinline__ void lane_accumulate(BlockAcc &acc, const LaneOut &o)
+{
+    const int lane = threadIdx.x & (kWave - 1);
     const double sx = wave_sum(o.Jxfer), st = wave_sum(o.Jtr), sl = wave_sum(o.JLCC), sr = wave_sum(o.Jrel);
     const int n_it = wave_sum(static_cast<int>(o.iters)), n_open = wave_sum(o.open), n_cl = wave_sum(o.clch);
     const int k = lane & 3;
@@ -608,6 +703,75 @@ __device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t st
         const int64_t iv = lane == 0 ? n_it : lane == 1 ? n_open : lane == 2 ? n_cl : (bad ? 1 : 0);
         atomicAdd(&acc.ints[lane], static_cast<unsigned long long>(iv));
     }
+}
+
+// The SSA of block b with one lane per CRU (FP32-local variant). Wave w of the
+// block owns kWave * CPT consecutive CRUs of the block's cell. Its lanes start
+// on the first kWave of them; a lane whose CRU has finished the step takes the
+// next CRU of the range not yet taken (the finishing lanes in lane order, by
+// a ballot), so the wave makes about (iterations of its range) / kWave trips
+// rather than CPT times the most iterations of any one CRU. Which lane runs
+// which CRU depends only on the CRUs' own iteration counts, the same on every
+// run. Each lane sums the outputs of its CRUs in FP64 as it finishes them; the
+// per-wave sums are a fixed xor butterfly, added exactly across waves and
+// blocks.
+template <int PRNG, int BLOCK, int CPT>
+__device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t step0, LccTableF &T, BlockAcc &acc,
+                                               LaneSums<BLOCK> &Ls)
+{
+    constexpr int kMap = shape_of_crus(BLOCK * CPT);
+    static_assert(crus_per_block(kMap) == BLOCK * CPT, "CRUs per block must be 16 << map");
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const int ks = static_cast<int>(step0 & 1);
+    const DevCell &C = a.cells[cell];
+    const CellDyn &D = a.dyn[cell];
+    const SsaParamsF &sp = C.spf;
+    const StepConstsF &kk = D.kf[ks];
+    const float dt = static_cast<float>(a.dt);
+    load_lcc_table(kk, T);
+    acc_zero(acc);
+    Ls.zero(tid);
+    __syncthreads();
+    const int64_t w0 = static_cast<int64_t>(b - C.blk0[kMap]) * (BLOCK * CPT) + wave * (kWave * CPT);
+    const int64_t w1 = min(w0 + kWave * CPT, static_cast<int64_t>(C.n_cru));
+    int64_t i = w0 + lane;
+    int64_t next = w0 + kWave;   // first CRU of the range not yet taken (wave-uniform)
+    bool have = (i < w1) && (D.status == 0);
+    LaneState S;
+    if (have) {
+        lane_load(a, C.slot0 + i, S);
+        lane_begin(S, sp);
+    }
+    while (__ballot(have)) {
+        bool fin = false;
+        if (have) {
+            fin = lane_iterate<PRNG>(S, sp, kk, T, dt);
+            if (fin) {
+                lane_store(a, C.slot0 + i, S);
+                Ls.add(tid, lane_outputs(S));
+            }
+        }
+        if constexpr (CPT > 1) {
+            const uint64_t m = __ballot(fin);
+            if (fin) {
+                const int rank = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+                i = next + rank;
+                have = i < w1;
+                if (have) {
+                    lane_load(a, C.slot0 + i, S);
+                    lane_begin(S, sp);
+                }
+            }
+            next += __popcll(m);
+        } else {
+            if (fin) have = false;
+        }
+    }
+    lane_accumulate(acc, Ls.get(tid));
     __syncthreads();
     if (tid == 0) {
         Partial p;
@@ -624,20 +788,21 @@ __device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t st
     }
 }
 
-// One lane per CRU needs about 100-128 VGPRs; 4 waves per SIMD keep it free of spills
-// and still hold 256 CRUs per SIMD.
-template <int PRNG, int BLOCK>
+// One lane per CRU needs about 100-128 VGPRs; 4 waves per SIMD keep it free
+// of spills and still hold 256 lanes per SIMD.
+template <int PRNG, int BLOCK, int CPT>
 __global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_step, int32_t nclose, const int64_t *base)
 {
     __shared__ LccTableF T;
-    __shared__ Partial red[kWave];
+    __shared__ CloseShared csh;
     __shared__ BlockAcc acc;
+    __shared__ LaneSums<BLOCK> Ls;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kWave>(a, bid, red);
+        close_cell<kWave>(a, bid, csh);
         return;
     }
-    ssa_block_lane<PRNG, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc);
+    ssa_block_lane<PRNG, BLOCK, CPT>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc, Ls);
 }
 
 // Graph bookkeeping: the step index a replayed chunk starts at.
@@ -788,17 +953,17 @@ struct gw_batch {
     int32_t trace_every = 0;
     int64_t trace_cap = 0;
     int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
-    int32_t lanes = 4;               // lanes per CRU: 4 (quad) or 1 (FP32-local only)
+    int32_t cfg = 2;                 // launch configuration (kConfigs), 4 x 256 by default
     // replayed graphs of step chunks (0 = plain launches)
     int32_t graph_steps = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int32_t graph_steps_built = 0, graph_lanes = 0, graph_shape = 0;
+    int32_t graph_steps_built = 0, graph_cfg = -1;
     bool graph_digest = false;
     KArgs graph_args{};
     int64_t *d_step_base = nullptr;
-    int64_t nblocks_shape[kNumShapes] = {0, 0, 0, 0};
-    int64_t bc_off[kNumShapes] = {0, 0, 0, 0};   // offset of each shape's block -> cell map
+    int64_t nblocks_shape[kNumShapes] = {};
+    int64_t bc_off[kNumShapes] = {};   // offset of each map's block -> cell table
     int64_t max_nblocks = 0;
     std::vector<DevCell> cells_h;
     std::vector<int32_t> block_cell_h;
@@ -842,33 +1007,50 @@ int launch_cfg_check(gw_batch *B, char *err, size_t errlen)
     return GW_OK;
 }
 
-void set_shape(gw_batch *B, int s, int lanes = 4)
+// The launch configurations: lanes per CRU (4: a quad per CRU, FP64 or
+// FP32-local; 1: one lane per CRU, FP32-local), threads per block, and CRUs
+// per lane (lane kernels only: each wave owns 64 x that many CRUs and refills
+// its lanes as CRUs finish their step).
+struct LaunchCfg {
+    int lanes, threads, cpl;
+};
+constexpr int kNumConfigs = GW_LAUNCH_CONFIGS;
+constexpr int kConfigDefault = 2;
+constexpr LaunchCfg kConfigs[kNumConfigs] = {
+    {4, 64, 1},  {4, 128, 1}, {4, 256, 1}, {4, 512, 1},                 // quads: 16..128 CRUs per block
+    {1, 64, 1},  {1, 128, 1}, {1, 256, 1},                               // one lane per CRU
+    {1, 64, 2},  {1, 128, 2}, {1, 256, 2},                               // two CRUs per lane, refilled
+    {1, 64, 4},  {1, 128, 4}, {1, 256, 4},                               // four
+    {1, 64, 8},  {1, 128, 8}, {1, 256, 8}};                              // eight (up to 2048 CRUs per block)
+constexpr int config_shape(int c)
 {
-    B->lanes = lanes;
-    B->shape = s;
-    B->nblocks = B->nblocks_shape[s];
-    B->a.shape = s;
-    B->a.nblocks = B->nblocks_shape[s];
-    B->a.block_cell = B->d_block_cell + B->bc_off[s];
+    return kConfigs[c].lanes == 4 ? shape_of_block(kConfigs[c].threads) : shape_of_crus(kConfigs[c].threads * kConfigs[c].cpl);
+}
+static_assert(config_shape(kConfigDefault) == kShapeDefault, "the default configuration uses the setup map");
+static_assert(config_shape(kNumConfigs - 1) < kNumShapes, "block -> cell maps cover every configuration");
+
+void set_shape(gw_batch *B, int c)
+{
+    B->cfg = c;
+    B->shape = config_shape(c);
+    B->nblocks = B->nblocks_shape[B->shape];
+    B->a.shape = B->shape;
+    B->a.nblocks = B->nblocks;
+    B->a.block_cell = B->d_block_cell + B->bc_off[B->shape];
 }
 
-// Shapes other than the default are built for the fast path only: FP64
-// without replay digests or FP32-local, xoshiro256**, free-running globals.
-bool shape_allowed(const gw_batch *B, int s, int lanes = 4)
+// Configurations other than the default are built for the fast path only:
+// FP64 without replay digests or FP32-local, xoshiro256**, free-running
+// globals; one lane per CRU is FP32-local only.
+bool shape_allowed(const gw_batch *B, int c)
 {
-    if (lanes == 1)   // one lane per CRU: FP32-local, 64 or 128 CRUs per block
-        return B->precision == GW_PREC_FP32_LOCAL && B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen && s >= 2;
-    return s == kShapeDefault || (B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen &&
-                                  (B->precision == GW_PREC_FP32_LOCAL || !B->digest));
+    const bool fast = B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen;
+    if (kConfigs[c].lanes == 1) return fast && B->precision == GW_PREC_FP32_LOCAL;
+    return c == kConfigDefault || (fast && (B->precision == GW_PREC_FP32_LOCAL || !B->digest));
 }
 
-// The launch configurations the autotuner tries: (lanes per CRU, threads per block).
-constexpr int kNumConfigs = 6;
 constexpr int kAutotuneGraphSteps = 64;   // steps per captured graph the autotuner tries
 constexpr int kAutotuneWarm = kAutotuneGraphSteps + 2;   // untimed steps before every trial
-constexpr int kConfigLanes[kNumConfigs] = {4, 4, 4, 4, 1, 1};
-constexpr int kConfigThreads[kNumConfigs] = {64, 128, 256, 512, 64, 128};
-int config_shape(int c) { return kConfigLanes[c] == 4 ? shape_of_block(kConfigThreads[c]) : (kConfigThreads[c] == 64 ? 2 : 3); }
 
 int prepare(gw_batch *B, char *err, size_t errlen)
 {
@@ -950,12 +1132,22 @@ template <int PRNG>
 void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hipEvent_t e1, const int64_t *base)
 {
     if constexpr (PRNG == GW_PRNG_XOSHIRO256SS) {
-        if (B->lanes == 1) {
+        if (kConfigs[B->cfg].lanes == 1) {
             const unsigned grid = step_grid(B, ssa_step, nclose);
-            if (B->shape == 2)
-                dispatch(k_step_lane<PRNG, 64>, grid, 64, B, e0, e1, ssa_step, nclose, base);
-            else
-                dispatch(k_step_lane<PRNG, 128>, grid, 128, B, e0, e1, ssa_step, nclose, base);
+            switch (B->cfg) {
+            case 4: dispatch(k_step_lane<PRNG, 64, 1>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 5: dispatch(k_step_lane<PRNG, 128, 1>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
+            case 6: dispatch(k_step_lane<PRNG, 256, 1>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
+            case 7: dispatch(k_step_lane<PRNG, 64, 2>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 8: dispatch(k_step_lane<PRNG, 128, 2>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
+            case 9: dispatch(k_step_lane<PRNG, 256, 2>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
+            case 10: dispatch(k_step_lane<PRNG, 64, 4>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 11: dispatch(k_step_lane<PRNG, 128, 4>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
+            case 12: dispatch(k_step_lane<PRNG, 256, 4>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
+            case 13: dispatch(k_step_lane<PRNG, 64, 8>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 14: dispatch(k_step_lane<PRNG, 128, 8>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
+            default: dispatch(k_step_lane<PRNG, 256, 8>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
+            }
             return;
         }
         switch (B->shape) {
@@ -986,8 +1178,8 @@ void launch_one(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0, hi
 // captured again whenever the launch arguments or shape change.
 bool graph_current(const gw_batch *B)
 {
-    return B->graph_exec && B->graph_steps_built == B->graph_steps && B->graph_lanes == B->lanes &&
-           B->graph_shape == B->shape && B->graph_digest == B->digest &&
+    return B->graph_exec && B->graph_steps_built == B->graph_steps && B->graph_cfg == B->cfg &&
+           B->graph_digest == B->digest &&
            std::memcmp(&B->graph_args, &B->a, sizeof(KArgs)) == 0;
 }
 
@@ -1017,8 +1209,7 @@ hipError_t graph_build(gw_batch *B)
         return e;
     }
     B->graph_steps_built = B->graph_steps;
-    B->graph_lanes = B->lanes;
-    B->graph_shape = B->shape;
+    B->graph_cfg = B->cfg;
     B->graph_digest = B->digest;
     B->graph_args = B->a;
     return hipSuccess;
@@ -1321,7 +1512,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         C.n_cru = d.n_cru;
         C.slot0 = slot;
         for (int sh = 0; sh < kNumShapes; ++sh) {
-            const int per = shape_threads(sh) / 4;
+            const int per = crus_per_block(sh);
             C.nblk[sh] = (d.n_cru + per - 1) / per;
         }
         slot += static_cast<int64_t>((d.n_cru + kSlotAlign - 1) / kSlotAlign) * kSlotAlign;
@@ -1682,39 +1873,49 @@ int gw_batch_get_iterations(gw_batch *B, uint64_t *iters)
     return GW_OK;
 }
 
-int gw_batch_set_launch(gw_batch *B, int32_t lanes_per_cru, int32_t block, char *err, size_t errlen)
+int gw_launch_config(int32_t index, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane)
+{
+    if (index < 0 || index >= kNumConfigs) return GW_E_INVALID_ARG;
+    if (lanes_per_cru) *lanes_per_cru = kConfigs[index].lanes;
+    if (block) *block = kConfigs[index].threads;
+    if (crus_per_lane) *crus_per_lane = kConfigs[index].cpl;
+    return GW_OK;
+}
+
+int gw_batch_set_launch(gw_batch *B, int32_t lanes_per_cru, int32_t block, int32_t crus_per_lane, char *err,
+                        size_t errlen)
 {
     if (!B) return GW_E_INVALID_ARG;
     const int lanes = lanes_per_cru ? lanes_per_cru : 4;
     const int threads = block ? block : (lanes == 4 ? kBlock : 64);
+    const int cpl = crus_per_lane ? crus_per_lane : 1;
     int c = -1;
     for (int k = 0; k < kNumConfigs; ++k)
-        if (kConfigLanes[k] == lanes && kConfigThreads[k] == threads) c = k;
+        if (kConfigs[k].lanes == lanes && kConfigs[k].threads == threads && kConfigs[k].cpl == cpl) c = k;
     if (c < 0) {
-        set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads is not available (4 lanes: 64, 128, 256 "
-                "or 512 threads; 1 lane: 64 or 128 threads)", lanes_per_cru, block);
+        set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads x %d CRUs per lane is not available "
+                "(4 lanes: 64, 128, 256 or 512 threads, 1 CRU per lane; 1 lane: 64, 128 or 256 threads, 1, 2, 4 or 8 "
+                "CRUs per lane)", lanes_per_cru, block, crus_per_lane);
         return GW_E_INVALID_ARG;
     }
-    if (!shape_allowed(B, config_shape(c), lanes)) {
+    if (!shape_allowed(B, c)) {
         set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads is not available for this batch (other "
                 "shapes than 4 x 256 need xoshiro256** without replay digests or frozen globals; 1 lane per CRU needs "
                 "FP32-local)", lanes, threads);
         return GW_E_INVALID_ARG;
     }
-    set_shape(B, config_shape(c), lanes);
+    set_shape(B, c);
     return GW_OK;
 }
 
-int gw_batch_get_launch(gw_batch *B, int32_t *lanes_per_cru, int32_t *block)
+int gw_batch_get_launch(gw_batch *B, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane)
 {
     if (!B) return GW_E_INVALID_ARG;
-    if (lanes_per_cru) *lanes_per_cru = B->lanes;
-    if (block) *block = B->lanes == 4 ? shape_threads(B->shape) : (B->shape == 2 ? 64 : 128);
-    return GW_OK;
+    return gw_launch_config(B->cfg, lanes_per_cru, block, crus_per_lane);
 }
 
-int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out, int32_t *graph_out,
-                      double *ms_per_step, char *err, size_t errlen)
+int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out, int32_t *cpl_out,
+                      int32_t *graph_out, double *ms_per_step, char *err, size_t errlen)
 {
     if (!B || trial_steps <= 0) {
         set_err(err, errlen, "gw_batch_autotune: bad arguments");
@@ -1759,12 +1960,12 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
     if (he == hipSuccess) he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
     const int64_t steps0 = B->steps_done;
-    int best_shape = B->shape, best_lanes = B->lanes;
+    int best_cfg = B->cfg;
     double best_ms = 1e300;
     for (int c = 0; c < kNumConfigs && he == hipSuccess; ++c) {
         if (ms_per_step) ms_per_step[c] = -1.0;
-        if (!shape_allowed(B, config_shape(c), kConfigLanes[c])) continue;
-        set_shape(B, config_shape(c), kConfigLanes[c]);
+        if (!shape_allowed(B, c)) continue;
+        set_shape(B, c);
         if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) != hipSuccess) break;
         B->steps_done += kAutotuneWarm;
         if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
@@ -1777,8 +1978,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         if (ms_per_step) ms_per_step[c] = ms;
         if (ms < best_ms) {
             best_ms = ms;
-            best_shape = config_shape(c);
-            best_lanes = kConfigLanes[c];
+            best_cfg = c;
         }
         he = copy_all(false);
         B->steps_done = steps0;
@@ -1788,7 +1988,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
     int32_t best_graph = 0;
     if (ms_per_step) ms_per_step[kNumConfigs] = -1.0;
     if (he == hipSuccess && trial_steps > kAutotuneGraphSteps) {
-        set_shape(B, best_shape, best_lanes);
+        set_shape(B, best_cfg);
         B->graph_steps = kAutotuneGraphSteps;
         // the warm-up captures and replays the graph once, outside the timed window
         if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) == hipSuccess) {
@@ -1810,7 +2010,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
     if (he == hipSuccess) he = copy_all(false);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     B->steps_done = steps0;
-    set_shape(B, best_shape, best_lanes);
+    set_shape(B, best_cfg);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(snap);
@@ -1818,7 +2018,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         set_err(err, errlen, "autotune: %s", hipGetErrorString(he));
         return GW_E_HIP;
     }
-    gw_batch_get_launch(B, lanes_out, block_out);
+    gw_batch_get_launch(B, lanes_out, block_out, cpl_out);
     if (graph_out) *graph_out = B->graph_steps;
     return check_status(B, err, errlen);
 }
@@ -1873,7 +2073,7 @@ int gw_batch_set_mode(gw_batch *B, int32_t frozen, int32_t digest)
     B->frozen = frozen != 0;
     B->a.frozen = frozen != 0;
     B->digest = digest != 0;
-    if (!shape_allowed(B, B->shape, B->lanes)) set_shape(B, kShapeDefault);
+    if (!shape_allowed(B, B->cfg)) set_shape(B, kConfigDefault);
     return GW_OK;
 }
 

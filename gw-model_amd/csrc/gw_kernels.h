@@ -9,14 +9,19 @@ constexpr int kBlock = 256;        // threads per block of the setup kernels (64
 constexpr int kWave = 64;
 constexpr int kCruPerBlock = kBlock / 4;  // one quad of lanes per CRU
 
-// SSA launch shapes: blocks of 64, 128, 256 or 512 threads (16..128 CRUs).
-// Every cell's CRU slots are padded to a multiple of the largest, so all
-// shapes address the same layout; the setup kernels use shape 2 (256).
-constexpr int kNumShapes = 4;
+// Block -> cell maps of the SSA launch shapes: map s gives blocks of 16 << s
+// CRUs (16..2048). A quad kernel with T threads uses map log2(T / 64); a lane
+// kernel with T threads and C CRUs per lane uses map log2(T * C / 16). Blocks
+// stop at the cell's last CRU, so the maps need no padding; every cell's CRU
+// slots start at a multiple of kSlotAlign for aligned, coalesced access. The
+// setup kernels use map 2 (256 threads, 64 CRUs).
+constexpr int kNumShapes = 8;
 constexpr int kShapeDefault = 2;
 constexpr int kSlotAlign = 128;
-__host__ __device__ constexpr int shape_threads(int s) { return 64 << s; }
+__host__ __device__ constexpr int crus_per_block(int s) { return 16 << s; }
+__host__ __device__ constexpr int shape_threads(int s) { return 64 << s; }   // quad kernels
 __host__ __device__ constexpr int shape_of_block(int threads) { return threads == 64 ? 0 : threads == 128 ? 1 : threads == 256 ? 2 : 3; }
+__host__ __device__ constexpr int shape_of_crus(int crus) { return crus <= 16 ? 0 : 1 + shape_of_crus(crus / 2); }
 
 // Static description of one cell on the device.
 struct DevCell {

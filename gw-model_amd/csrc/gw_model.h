@@ -81,21 +81,31 @@ struct StepConsts {
 };
 
 // GW::alphaLCC .. tauLCC (GW_utils.hpp:365-375)
-GW_HD double alphaLCC(double V) { return 2.0 * gw::exp(0.012 * (V - 35.0)); }
-GW_HD double betaLCC(double V) { return 0.0882 * gw::exp(-0.05 * (V - 35.0)); }
-GW_HD double yinfLCC(double V) { return 0.4 / (1.0 + gw::exp((V + 12.5) / 5.0)) + 0.6; }
-GW_HD double tauLCC(double V) { return 340.0 / (1.0 + gw::exp((V + 30.0) / 12.0)) + 60.0; }
+// The exponentials of the per-step constants: alpha, beta, y_inf and tau of
+// the LCC (GW_utils.hpp:365-375, as exp(arg)), then exp(-V F/RT).
+enum ScSlot : int { SC_AL, SC_BE, SC_YINF, SC_TAU, SC_EXPM, SC_N };
+
+GW_HD void sc_args(double V, double F_RT, double *arg)
+{
+    arg[SC_AL] = 0.012 * (V - 35.0);
+    arg[SC_BE] = -0.05 * (V - 35.0);
+    arg[SC_YINF] = (V + 12.5) / 5.0;
+    arg[SC_TAU] = (V + 30.0) / 12.0;
+    arg[SC_EXPM] = -(V * F_RT);
+}
 
 // Per-step constants of GW_model::SSA (GW.tpp:216-221) and the LCC rate table
-// of update_LCC_rates (GW_utils.tpp:118-186). At V == 0 the multiplier takes
-// its analytic limit JLCC_const / 2 instead of 0/0.
-GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, double expmVF_RT,
-                       double Cai, double CaNSR, uint32_t step_lo)
+// of update_LCC_rates (GW_utils.tpp:118-186), from the exponentials X[SC_*]
+// of sc_args. At V == 0 the multiplier takes its analytic limit
+// JLCC_const / 2 instead of 0/0.
+GW_HD void step_consts_x(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, const double *X,
+                         double Cai, double CaNSR, uint32_t step_lo)
 {
-    const double al = alphaLCC(V);
-    const double be = betaLCC(V);
-    const double yinf = yinfLCC(V);
-    const double tau = tauLCC(V);
+    const double expmVF_RT = X[SC_EXPM];
+    const double al = 2.0 * X[SC_AL];
+    const double be = 0.0882 * X[SC_BE];
+    const double yinf = 0.4 / (1.0 + X[SC_YINF]) + 0.6;
+    const double tau = 340.0 / (1.0 + X[SC_TAU]) + 60.0;
     k.Cai = Cai;
     k.CaNSR = CaNSR;
     k.inact0 = yinf / tau;
@@ -129,6 +139,17 @@ GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c,
         else if (s >= 2 && s <= 5) k.lcc.P[s] = r0[s] + r1[s];
         else k.lcc.P[s] = r0[s] + r1[s] + r2[s];
     }
+}
+
+// step_consts_x with its exponentials evaluated in turn (exp(-V F/RT) given).
+GW_HD void step_consts(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, double expmVF_RT,
+                       double Cai, double CaNSR, uint32_t step_lo)
+{
+    double X[SC_N];
+    sc_args(V, c.F_RT, X);
+    for (int i = 0; i < SC_EXPM; ++i) X[i] = gw::exp(X[i]);
+    X[SC_EXPM] = expmVF_RT;
+    step_consts_x(k, p, c, V, X, Cai, CaNSR, step_lo);
 }
 
 // The SSA constants of a step rounded to FP32, for the FP32-local SSA.
@@ -168,58 +189,131 @@ struct FluxSums {
     double JLCC, Jxfer, Jtr, ClCh;
 };
 
-GW_HD double nernst(double Xi, double Xo, double RT_F, double z, bool &ok)
+// The exponentials, logarithms and powers of global_derivatives whose
+// arguments depend on the state alone, in a fixed slot order. gd_args forms
+// each argument exactly as the formula that uses it; gd_finish reads the
+// values X[slot]. The values may come from one evaluation after another
+// (global_derivatives) or from many lanes at once: the same function of the
+// same argument either way, so the derivatives are the same bits.
+enum GdSlot : int {
+    GD_EXPM,                                  // exp(-VF_RT)
+    GD_NACA, GD_NAK, GD_KR_R, GD_KP,          // INaCa, INaK, IKr rectification, IKp
+    GD_AM, GD_BM, GD_AH, GD_BH1, GD_BH2, GD_BH3, GD_AJ1, GD_AJ2, GD_AJ3,
+    GD_BJ1, GD_BJ2, GD_BJ3, GD_BJ4, GD_XKS, GD_TXKS1, GD_TXKS2,   // gates (common.hpp:175-218)
+    GD_KR0, GD_KR1, GD_KR2, GD_KR3, GD_KR4, GD_KR5, GD_KR6,       // update_QKr (GW.tpp:25-44)
+    GD_KV0, GD_KV1, GD_KV2, GD_KV3, GD_KV4, GD_KV5, GD_KV6, GD_KV7,   // update_QKv (GW.tpp:64-123)
+    GD_LOG0,                                  // logarithms from here
+    GD_LNA = GD_LOG0, GD_LK, GD_LKS,          // log(Nao/Nai), log(Ko/Ki), the IKs reversal
+    GD_POW0,                                  // powers from here (exponent py[slot - GD_POW0])
+    GD_PNAK = GD_POW0, GD_PF, GD_PR,          // INaK, Jup forward and reverse
+    GD_N
+};
+
+GW_HD double gd_eval(int slot, double arg, double y)
 {
-    if (Xi <= 0.0 || Xo <= 0.0) {   // the reference exits here (common.hpp:72-75)
+    return slot < GD_LOG0 ? gw::exp(arg) : slot < GD_POW0 ? gw::log(arg) : gw::pow(arg, y);
+}
+
+GW_HD void gd_args(const gw_params_t &p, const gw_globals_t &g, double VF_RT, double *arg, double *py)
+{
+    const double V = g.V;
+    arg[GD_EXPM] = -VF_RT;
+    arg[GD_NACA] = p.eta * VF_RT;
+    arg[GD_NAK] = -0.1 * VF_RT;
+    arg[GD_KR_R] = 0.0446 * V;
+    arg[GD_KP] = (7.488 - V) / 5.98;
+    arg[GD_AM] = -0.1 * (V + 47.13);
+    arg[GD_BM] = -V / 11.0;
+    arg[GD_AH] = -(V + 80.0) / 6.8;
+    arg[GD_BH1] = -(V + 10.66) / 11.1;
+    arg[GD_BH2] = 0.079 * V;
+    arg[GD_BH3] = 0.35 * V;
+    arg[GD_AJ1] = 0.2444 * V;
+    arg[GD_AJ2] = -0.04391 * V;
+    arg[GD_AJ3] = 0.311 * (V + 79.23);
+    arg[GD_BJ1] = -2.535e-7 * V;
+    arg[GD_BJ2] = -0.1 * (V + 32.0);
+    arg[GD_BJ3] = -0.01052 * V;
+    arg[GD_BJ4] = -0.1378 * (V + 40.14);
+    arg[GD_XKS] = -(V - 24.7) / 13.6;
+    arg[GD_TXKS1] = -0.148 * (V - 10.0);
+    arg[GD_TXKS2] = 0.0687 * (V - 10.0);
+    arg[GD_KR0] = 0.0272 * V;
+    arg[GD_KR1] = -0.0431 * V;
+    arg[GD_KR2] = 0.0262 * V;
+    arg[GD_KR3] = 2.71e-6 * V;
+    arg[GD_KR4] = -0.0269 * V;
+    arg[GD_KR5] = 0.0120 * V;
+    arg[GD_KR6] = -0.0443 * V;
+    arg[GD_KV0] = p.aaKv14 * V;
+    arg[GD_KV1] = p.aaKv43 * V;
+    arg[GD_KV2] = -p.aiKv14 * V;
+    arg[GD_KV3] = -p.aiKv43 * V;
+    arg[GD_KV4] = -p.baKv14 * V;
+    arg[GD_KV5] = -p.baKv43 * V;
+    arg[GD_KV6] = p.biKv14 * V;
+    arg[GD_KV7] = p.biKv43 * V;
+    arg[GD_LNA] = p.Nao / g.Nai;
+    arg[GD_LK] = p.Ko / g.Ki;
+    arg[GD_LKS] = (p.Ko + 0.01833 * p.Nao) / (g.Ki + 0.01833 * g.Nai);
+    arg[GD_PNAK] = p.KmNai / g.Nai;
+    py[0] = 1.5;
+    arg[GD_PF] = g.Cai / p.Kmf;
+    py[1] = p.Hf;
+    arg[GD_PR] = g.CaNSR / p.Kmr;
+    py[2] = p.Hr;
+}
+
+// Nernst potential (common.hpp:70-75) from lg = log(Xo / Xi); the reference
+// exits on a non-positive concentration, this clears ok instead.
+GW_HD double nernst(double Xi, double Xo, double lg, double RT_F, double z, bool &ok)
+{
+    if (Xi <= 0.0 || Xo <= 0.0) {
         ok = false;
         return 0.0;
     }
-    return gw::log(Xo / Xi) * RT_F / z;
+    return lg * RT_F / z;
 }
 
-GW_HD double alpha_m(double V) { return (V == -47.13) ? 3.2 : 0.32 * (V + 47.13) / (1.0 - gw::exp(-0.1 * (V + 47.13))); }
-GW_HD double beta_m(double V) { return 0.08 * gw::exp(-V / 11.0); }
-GW_HD double alpha_h(double V) { return (V >= -40.0) ? 0.0 : 0.135 * gw::exp(-(V + 80.0) / 6.8); }
-GW_HD double beta_h(double V)
+// Gate rates (common.hpp:175-218) from the exponentials of gd_args.
+GW_HD double alpha_m(double V, const double *X) { return (V == -47.13) ? 3.2 : 0.32 * (V + 47.13) / (1.0 - X[GD_AM]); }
+GW_HD double beta_m(const double *X) { return 0.08 * X[GD_BM]; }
+GW_HD double alpha_h(double V, const double *X) { return (V >= -40.0) ? 0.0 : 0.135 * X[GD_AH]; }
+GW_HD double beta_h(double V, const double *X)
 {
-    return (V >= -40.0) ? 1.0 / (0.13 * (1.0 + gw::exp(-(V + 10.66) / 11.1)))
-                        : 3.56 * gw::exp(0.079 * V) + 3.1e5 * gw::exp(0.35 * V);
+    return (V >= -40.0) ? 1.0 / (0.13 * (1.0 + X[GD_BH1])) : 3.56 * X[GD_BH2] + 3.1e5 * X[GD_BH3];
 }
-GW_HD double alpha_j(double V)
+GW_HD double alpha_j(double V, const double *X)
 {
-    return (V >= -40.0) ? 0.0
-                        : (-127140.0 * gw::exp(0.2444 * V) - 3.474e-5 * gw::exp(-0.04391 * V)) * (V + 37.78) /
-                              (1.0 + gw::exp(0.311 * (V + 79.23)));
+    return (V >= -40.0) ? 0.0 : (-127140.0 * X[GD_AJ1] - 3.474e-5 * X[GD_AJ2]) * (V + 37.78) / (1.0 + X[GD_AJ3]);
 }
-GW_HD double beta_j(double V)
+GW_HD double beta_j(double V, const double *X)
 {
-    return (V >= -40.0) ? 0.3 * gw::exp(-2.535e-7 * V) / (1.0 + gw::exp(-0.1 * (V + 32.0)))
-                        : 0.1212 * gw::exp(-0.01052 * V) / (1.0 + gw::exp(-0.1378 * (V + 40.14)));
+    return (V >= -40.0) ? 0.3 * X[GD_BJ1] / (1.0 + X[GD_BJ2]) : 0.1212 * X[GD_BJ3] / (1.0 + X[GD_BJ4]);
 }
-GW_HD double XKsinf(double V) { return 1.0 / (1.0 + gw::exp(-(V - 24.7) / 13.6)); }
-GW_HD double tauXKs_inv(double V)
+GW_HD double XKsinf(const double *X) { return 1.0 / (1.0 + X[GD_XKS]); }
+GW_HD double tauXKs_inv(double V, const double *X)
 {
     if (V == 10.0) return 0.0000719 / 0.148 + 0.000131 / 0.0687;  // analytic limit of 0/0
-    return 0.0000719 * (V - 10.0) / (1.0 - gw::exp(-0.148 * (V - 10.0))) +
-           0.000131 * (V - 10.0) / (gw::exp(0.0687 * (V - 10.0)) - 1.0);
+    return 0.0000719 * (V - 10.0) / (1.0 - X[GD_TXKS1]) + 0.000131 * (V - 10.0) / (X[GD_TXKS2] - 1.0);
 }
 
 // IKr chain: generator entries of update_QKr (GW.tpp:25-44) and the
 // derivatives of update_Kr_derivatives (GW.tpp:126-133), in registers.
-GW_HD void kr_chain(const gw_params_t &p, double V, const double x[5], double dt, double d[5])
+GW_HD void kr_chain(const gw_params_t &p, const double *X, const double x[5], double dt, double d[5])
 {
     const double q12 = p.Kf, q21 = p.Kb;   // initialise_QKr (GW.hpp:75)
-    const double q01 = 0.0069 * gw::exp(0.0272 * V);
+    const double q01 = 0.0069 * X[GD_KR0];
     const double q00 = -q01;
-    const double q10 = 0.0227 * gw::exp(-0.0431 * V);
+    const double q10 = 0.0227 * X[GD_KR1];
     const double q11 = -(q10 + q12);
-    const double q23 = 0.0218 * gw::exp(0.0262 * V);
-    const double q24 = 1.29e-5 * gw::exp(2.71e-6 * V);
+    const double q23 = 0.0218 * X[GD_KR2];
+    const double q24 = 1.29e-5 * X[GD_KR3];
     const double q22 = -(q21 + q23 + q24);
-    const double q32 = 0.0009 * gw::exp(-0.0269 * V);
-    const double q34 = 0.0622 * gw::exp(0.0120 * V);
+    const double q32 = 0.0009 * X[GD_KR4];
+    const double q34 = 0.0622 * X[GD_KR5];
     const double q33 = -(q32 + q34);
-    const double q43 = 0.0059 * gw::exp(-0.0443 * V);
+    const double q43 = 0.0059 * X[GD_KR6];
     const double q42 = q32 * q43 * q24 / (q23 * q34);
     const double q44 = -(q42 + q43);
     d[0] = dt * (q10 * x[1] + q00 * x[0]);
@@ -261,34 +355,36 @@ struct Currents {
     double INa, INab, INaCa, INaK, IKr, IKs, Ito1, Ito2, IK1, IKp, ICaL, ICab, IpCa, Jup, Jtr_tot, Jxfer_tot;
 };
 
-GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT,
-                             double expmVF_RT, double dt, double Istim, const FluxSums &S, gw_globals_t &dg,
-                             Currents &cur)
+// The derivatives from the values X[slot] of gd_args' arguments; the one
+// exponential whose argument needs a logarithm (IK1) is evaluated here.
+GW_HD int gd_finish(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT, const double *X,
+                    double dt, double Istim, const FluxSums &S, gw_globals_t &dg, Currents &cur)
 {
     const double V = g.V;
+    const double expmVF_RT = X[GD_EXPM];
     bool ok = true;
-    const double ENa = nernst(g.Nai, p.Nao, c.RT_F, 1.0, ok);
-    const double EK = nernst(g.Ki, p.Ko, c.RT_F, 1.0, ok);
-    const double ECa = nernst(g.Ki, p.Ko, c.RT_F, 2.0, ok);  // K-based, as GW.tpp:176
+    const double ENa = nernst(g.Nai, p.Nao, X[GD_LNA], c.RT_F, 1.0, ok);
+    const double EK = nernst(g.Ki, p.Ko, X[GD_LK], c.RT_F, 1.0, ok);
+    const double ECa = nernst(g.Ki, p.Ko, X[GD_LK], c.RT_F, 2.0, ok);  // K-based, as GW.tpp:176
     if (!ok) return GW_E_NONPOSITIVE_CONC;
 
     cur.INa = p.GNa * cube(g.m) * g.h * g.j * (V - ENa);                       // common.hpp:89-92
     cur.INab = p.GNab * (V - ENa);                                             // common.hpp:100-101
     {   // common.hpp:163-168
-        const double e1 = gw::exp(p.eta * VF_RT);
+        const double e1 = X[GD_NACA];
         const double e2 = e1 * expmVF_RT;
         cur.INaCa = c.INaCa_const * (e1 * cube(g.Nai) * p.Cao - e2 * c.Nao3 * g.Cai) / (1.0 + p.ksat * e2);
     }
     {   // common.hpp:121-124, 140-143
-        const double fNaK = 1.0 / (1.0 + 0.1245 * gw::exp(-0.1 * VF_RT) + 0.0365 * c.sigma * expmVF_RT);
-        cur.INaK = c.INaK_const * fNaK / (1.0 + gw::pow(p.KmNai / g.Nai, 1.5));
+        const double fNaK = 1.0 / (1.0 + 0.1245 * X[GD_NAK] + 0.0365 * c.sigma * expmVF_RT);
+        cur.INaK = c.INaK_const * fNaK / (1.0 + X[GD_PNAK]);
     }
     {   // GW_utils.hpp:273-279
-        const double RKr = 1.0 / (1.0 + 1.4945 * gw::exp(0.0446 * V));
+        const double RKr = 1.0 / (1.0 + 1.4945 * X[GD_KR_R]);
         cur.IKr = p.GKr * c.sqrtKo * RKr * g.Kr[3] * (V - EK) * 0.5;
     }
     {   // GW_utils.hpp:282-291
-        const double EKs = gw::log((p.Ko + 0.01833 * p.Nao) / (g.Ki + 0.01833 * g.Nai)) * c.RT_F;
+        const double EKs = X[GD_LKS] * c.RT_F;
         cur.IKs = p.GKs * sq(g.xKs) * (V - EKs);
     }
     double IKv14;
@@ -308,15 +404,15 @@ GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw
         cur.IK1 = p.GK1 * K1inf * c.IK1_const * (V - EK);
     }
     {   // GW_utils.hpp:315-319
-        const double Kp = 1. / (1. + gw::exp((7.488 - V) / 5.98));
+        const double Kp = 1. / (1. + X[GD_KP]);
         cur.IKp = p.GKp * Kp * (V - EK);
     }
     cur.ICaL = S.JLCC * c.ICaL_const;                                           // GW_utils.hpp:322-325
     cur.ICab = p.GCab * (V - ECa);
     cur.IpCa = p.IpCamax * (g.Cai / (p.KmpCa + g.Cai));                         // common.hpp:109-110
     {   // GW_utils.hpp:339-344
-        const double f = gw::pow(g.Cai / p.Kmf, p.Hf);
-        const double r = gw::pow(g.CaNSR / p.Kmr, p.Hr);
+        const double f = X[GD_PF];
+        const double r = X[GD_PR];
         cur.Jup = (p.Vmaxf * f - p.Vmaxr * r) / (1.0 + f + r);
     }
     cur.Jtr_tot = S.Jtr * c.CRU_factor;                                         // GW_utils.hpp:354-355
@@ -340,27 +436,37 @@ GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw
     dg.V = dt * (Istim - (cur.INa + cur.ICaL + cur.IKr + cur.IKs + cur.Ito1 + cur.IK1 + cur.IKp + cur.Ito2 +
                           cur.INaK + cur.INaCa + cur.IpCa + cur.ICab + cur.INab));
     // GW.tpp:162-168
-    dg.m = dt * (alpha_m(V) * (1.0 - g.m) - beta_m(V) * g.m);
-    dg.h = dt * (alpha_h(V) * (1.0 - g.h) - beta_h(V) * g.h);
-    dg.j = dt * (alpha_j(V) * (1.0 - g.j) - beta_j(V) * g.j);
-    dg.xKs = dt * (XKsinf(V) - g.xKs) * tauXKs_inv(V);
+    dg.m = dt * (alpha_m(V, X) * (1.0 - g.m) - beta_m(X) * g.m);
+    dg.h = dt * (alpha_h(V, X) * (1.0 - g.h) - beta_h(V, X) * g.h);
+    dg.j = dt * (alpha_j(V, X) * (1.0 - g.j) - beta_j(V, X) * g.j);
+    dg.xKs = dt * (XKsinf(X) - g.xKs) * tauXKs_inv(V, X);
     // GW.tpp:25-44, 64-160: the IKr and Kv chains
-    kr_chain(p, V, g.Kr, dt, dg.Kr);
+    kr_chain(p, X, g.Kr, dt, dg.Kr);
     {
-        const double aa14 = p.alphaa0Kv14 * gw::exp(p.aaKv14 * V);
-        const double aa43 = p.alphaa0Kv43 * gw::exp(p.aaKv43 * V);
-        const double ai14 = p.alphai0Kv14 * gw::exp(-p.aiKv14 * V);
-        const double ai43 = p.alphai0Kv43 * gw::exp(-p.aiKv43 * V);
-        const double ba14 = p.betaa0Kv14 * gw::exp(-p.baKv14 * V);
-        const double ba43 = p.betaa0Kv43 * gw::exp(-p.baKv43 * V);
-        const double bi14 = p.betai0Kv14 * gw::exp(p.biKv14 * V);
-        const double bi43 = p.betai0Kv43 * gw::exp(p.biKv43 * V);
+        const double aa14 = p.alphaa0Kv14 * X[GD_KV0];
+        const double aa43 = p.alphaa0Kv43 * X[GD_KV1];
+        const double ai14 = p.alphai0Kv14 * X[GD_KV2];
+        const double ai43 = p.alphai0Kv43 * X[GD_KV3];
+        const double ba14 = p.betaa0Kv14 * X[GD_KV4];
+        const double ba43 = p.betaa0Kv43 * X[GD_KV5];
+        const double bi14 = p.betai0Kv14 * X[GD_KV6];
+        const double bi43 = p.betai0Kv43 * X[GD_KV7];
         kv_chain(aa14, ai14, ba14, bi14, p.f1Kv14, p.f2Kv14, p.f3Kv14, p.f4Kv14, p.b1Kv14, p.b2Kv14, p.b3Kv14,
                  p.b4Kv14, g.Kv14, dt, dg.Kv14);
         kv_chain(aa43, ai43, ba43, bi43, p.f1Kv43, p.f2Kv43, p.f3Kv43, p.f4Kv43, p.b1Kv43, p.b2Kv43, p.b3Kv43,
                  p.b4Kv43, g.Kv43, dt, dg.Kv43);
     }
     return GW_OK;
+}
+
+// gd_finish with gd_args' functions evaluated one after another.
+GW_HD int global_derivatives(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT,
+                             double dt, double Istim, const FluxSums &S, gw_globals_t &dg, Currents &cur)
+{
+    double X[GD_N], py[GD_N - GD_POW0];
+    gd_args(p, g, VF_RT, X, py);
+    for (int i = 0; i < GD_N; ++i) X[i] = gd_eval(i, X[i], i >= GD_POW0 ? py[i - GD_POW0] : 0.0);
+    return gd_finish(p, c, g, VF_RT, X, dt, Istim, S, dg, cur);
 }
 
 // globals += dGlobals (GW.tpp:252-271); V is held when clamped.

@@ -264,6 +264,86 @@ __device__ __forceinline__ R qsum4(R v) { return qget<0>(v) + qget<1>(v) + qget<
 __device__ __forceinline__ int qsum4i(int v) { return qgeti<0>(v) + qgeti<1>(v) + qgeti<2>(v) + qgeti<3>(v); }
 
 // Replicated CRU-level state of a quad (CaJSR accumulates in FP64 in both forms).
+// sample_RyR's choice (SSA.tpp:200-259): the first of the 12 transitions
+// whose cumulative rate reaches uu. In FP64 this is the reference's rule as
+// it stands. In FP32 the scaled uniform can exceed the cumulative sum by
+// rounding, which would take the last transition whatever its rate; there
+// only transitions with a positive rate are taken (the last of them on such
+// an excess, none (-1) if no rate is positive), so no RyR leaves an empty
+// state.
+template <class R>
+__device__ __forceinline__ int ryr_pick(const SsaParamsT<R> &P, uint64_t w, R ca, R uu)
+{
+    int tr = sizeof(R) == 8 ? 0 : -1;
+    R cum = R(0);
+    bool done = false;
+    ryr_rates(P, w, ca, [&](int k, R r) {
+        if (done) return;
+        if (sizeof(R) == 8) {
+            if (k == 0) cum = r;
+            else if (cum < uu) { tr = k; cum += r; }
+            else done = true;
+        } else {
+            if (k == 0) {
+                cum = r;
+                if (r > R(0)) tr = 0;
+            } else if (cum < uu) {
+                if (r > R(0)) tr = k;
+                cum += r;
+            } else {
+                done = true;
+            }
+        }
+    });
+    return tr;
+}
+
+// Source state (0..5) of RyR transition tr (0..11, in sample_RyR's order).
+GW_HD int ryr_from(int tr)
+{
+    return (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
+         : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
+}
+
+// FP32-local only: whether transition tr may be applied to word w at the
+// subspace Ca the step has just reached. The transition is chosen from the
+// propensities before the Ca update (as in the reference); when that update
+// re-split a lumped pair, a transition chosen at a pair-sum rate can take a
+// member below zero outside lumped mode, and negative counts make negative
+// propensities (a negative waiting time runs the Euler updates backwards).
+// Such a transition is not applied: a member leaves only while it holds a
+// RyR, or while its pair is lumped and the pair holds one.
+GW_HD bool ryr_allowed(uint64_t w, int tr, double ca)
+{
+    const int from = ryr_from(tr);
+    if ((from == 2 || from == 3) && ca > 0.03685) return ryr_get(w, 2) + ryr_get(w, 3) > 0;
+    if ((from == 4 || from == 5) && ca > 1.15e-4) return ryr_get(w, 4) + ryr_get(w, 5) > 0;
+    return ryr_get(w, from) > 0;
+}
+
+// Re-splits due in update_CaSS (SSA.tpp:93-96) as a mask: 1 for the (5,6)
+// pair, 2 for the (3,4) pair. The reference re-splits at most one pair per
+// update (the (5,6) pair when Ca falls through both thresholds at once); the
+// FP32-local form re-splits both then, so no lumped pair is left unsplit
+// below its threshold.
+template <class R>
+GW_HD int split_mask(double oldA, double tmp)
+{
+    const int m56 = (oldA > 1.15e-4 && tmp <= 1.15e-4) ? 1 : 0;
+    const int m34 = (oldA > 0.03685 && tmp <= 0.03685) ? 2 : 0;
+    if (sizeof(R) == 8) return m56 ? m56 : m34;
+    return m56 | m34;
+}
+
+// Apply RyR transition tr (0..11, in sample_RyR's order) to a subunit word.
+GW_HD uint64_t ryr_apply(uint64_t w, int tr)
+{
+    const int from = ryr_from(tr);
+    const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
+                 : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
+    return ryr_add(ryr_add(w, from, -1), to, +1);
+}
+
 struct CruQ {
     double jsr;
     uint64_t r0, r1, r2, r3;
@@ -336,18 +416,19 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
         const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old)) + ldiv(P.BSL_const, sq(P.KBSL + old)));
         const R dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
         const double tmp = oldA + static_cast<double>(dCaSS);
-        const int split = (oldA > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((oldA > 0.03685 && tmp <= 0.03685) ? 2 : 0);
+        const int split = split_mask<R>(oldA, tmp);
         const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
         if (any_split) {
             // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
             // lane draws the same Bernoulli uniforms and the owner applies them
             for (int jj = 0; jj < 4; ++jj) {
                 const int sj = qseli(split, jj);
-                if (sj) {
-                    const int a_ = (sj == 1) ? 4 : 2;
+                for (int bit = 1; bit <= 2; bit <<= 1) {
+                    if (!(sj & bit)) continue;
+                    const int a_ = (bit == 1) ? 4 : 2;
                     const uint64_t wq = qselu(w, jj);
                     const R oq = qsel(old, jj);
-                    const R pr = (sj == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(oq)) : ldiv(P.k43, P.k43 + P.k34 * sq(oq));
+                    const R pr = (bit == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(oq)) : ldiv(P.k43, P.k43 + P.k34 * sq(oq));
                     const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
                     int X = 0;
                     for (int i = 0; i < n; ++i)
@@ -355,7 +436,7 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
                     const int na = (n > 0) ? X : 0;
                     const int nb = (n > 0) ? n - X : 0;
                     if (j == jj) w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
-                    if (DIGEST) digest_event(dig, step_lo, jj, sj == 1 ? 4 : 5, na);
+                    if (DIGEST) digest_event(dig, step_lo, jj, bit == 1 ? 4 : 5, na);
                 }
             }
         }
@@ -423,20 +504,9 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
                 detail = yn;
             } else if (cls == 2) {
                 // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
-                const R uu = u4 * sum_RyR;
-                int tr = 0;
-                R cum = R(0);
-                bool done = false;
-                ryr_rates(P, w_pre, old, [&](int k, R r) {
-                    if (done) return;
-                    if (k == 0) { cum = r; }
-                    else { if (cum < uu) { tr = k; cum += r; } else done = true; }
-                });
-                const int from = (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
-                               : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
-                const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
-                             : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
-                w = ryr_add(ryr_add(w, from, -1), to, +1);
+                int tr = ryr_pick(P, w_pre, old, u4 * sum_RyR);
+                if (sizeof(R) == 4 && tr >= 0 && !ryr_allowed(w, tr, ca)) tr = -1;
+                if (tr >= 0) w = ryr_apply(w, tr);
                 if (tr == 1 || tr == 10) open += 1;
                 else if (tr == 4 || tr == 7) open -= 1;
                 Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
@@ -482,166 +552,193 @@ struct LaneOut {
     int err;
 };
 
-template <int PRNG>
-__device__ LaneOut ssa_step_lane(double (&ca)[4], float (&jl)[4], uint64_t (&w)[4], CruQ &q, const SsaParamsF &P,
-                                 const StepConstsF &K, const LccTableF &T, float dt)
+// The working state of one CRU during one global step.
+struct LaneState {
+    double ca[4];
+    uint64_t w[4];
+    CruQ q;
+    float jl[4];
+    float Jrel[4];
+    float Jxfer[4];
+    int open[4];
+    float t, Jtr;
+    uint32_t iters;
+    int err;
+};
+
+// copy_from_CRUState (SSA.tpp:10-33): the step's starting fluxes.
+__device__ __forceinline__ void lane_begin(LaneState &S, const SsaParamsF &P)
 {
     using R = float;
-    LaneOut out;
-    out.iters = 0;
-    out.err = 0;
-    const double Cai = K.Cai_d, CaNSR = K.CaNSR_d;
-    // copy_from_CRUState (SSA.tpp:10-33)
-    int open[4];
-    R Jrel[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        open[j] = ryr_get(w[j], 2) + ryr_get(w[j], 3);
-        Jrel[j] = P.rRyR * static_cast<R>(open[j]) * static_cast<R>(q.jsr - ca[j]);
+        S.open[j] = ryr_get(S.w[j], 2) + ryr_get(S.w[j], 3);
+        S.Jrel[j] = P.rRyR * static_cast<R>(S.open[j]) * static_cast<R>(S.q.jsr - S.ca[j]);
+        S.Jxfer[j] = R(0);
     }
-    R t = R(0), Jtr = R(0);
-    R Jxfer[4] = {R(0), R(0), R(0), R(0)};
-    while (true) {
-        out.iters++;
-        // update_fluxes + update_rates (SSA.tpp:37-56)
-        double oldA[4];
-        R old[4], s[4];
-        uint64_t w_pre[4];
+    S.t = R(0);
+    S.Jtr = R(0);
+    S.iters = 0;
+    S.err = 0;
+}
+
+// One iteration of SSA_single_CRU's loop (SSA.tpp:290-309); true when the
+// step is finished. The event (subunit, class, transition) is chosen from the
+// propensities before this iteration's Ca update, as in the reference, but
+// its uniforms are drawn before the re-split Bernoulli draws of that update
+// rather than after them: the draws are independent, so the joint law of the
+// choice and the re-splits is the same, and only the choice (not the
+// pre-update channel words and totals) has to be kept across the update.
+template <int PRNG>
+__device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, const StepConstsF &K,
+                                             const LccTableF &T, float dt)
+{
+    using R = float;
+    const double Cai = K.Cai_d, CaNSR = K.CaNSR_d;
+    S.iters++;
+    // update_fluxes + update_rates (SSA.tpp:37-56)
+    double oldA[4];
+    R old[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            oldA[j] = ca[j];
-            old[j] = static_cast<R>(oldA[j]);
-            w_pre[j] = w[j];
-        }
-        Jtr = P.rtr * static_cast<R>(CaNSR - q.jsr);
+    for (int j = 0; j < 4; ++j) {
+        oldA[j] = S.ca[j];
+        old[j] = static_cast<R>(oldA[j]);
+    }
+    S.Jtr = P.rtr * static_cast<R>(CaNSR - S.q.jsr);
+    int js = 0, cls = 0, tr = 0;
+    R jump, h;
+    bool last;
+    {
+        R s[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, w[j], old[j]);
+        for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, S.w[j], old[j]);
         const R total = s[0] + s[1] + s[2] + s[3];
-        const R jump = ldiv(lneglog(qdraw<PRNG, R>(q)), total);
-        const bool last = !(t + jump < dt);
-        const R h = last ? dt - t : jump;
-        // update_CaSS (SSA.tpp:87-99): subunits in order, re-splits drawn as they come
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const R Jiss = P.riss * static_cast<R>(oldA[(j + 1) & 3] + oldA[(j + 3) & 3] - 2 * oldA[j]);
-            Jxfer[j] = P.rxfer * static_cast<R>(oldA[j] - Cai);
-            const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old[j])) + ldiv(P.BSL_const, sq(P.KBSL + old[j])));
-            const R dCaSS = h * (jl[j] + Jrel[j] - Jxfer[j] + Jiss) * betaSS;
-            const double tmp = oldA[j] + static_cast<double>(dCaSS);
-            const int split = (oldA[j] > 1.15e-4 && tmp <= 1.15e-4) ? 1 : ((oldA[j] > 0.03685 && tmp <= 0.03685) ? 2 : 0);
-            if (split) {
-                // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84)
-                const int a_ = (split == 1) ? 4 : 2;
-                const R pr = (split == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(old[j])) : ldiv(P.k43, P.k43 + P.k34 * sq(old[j]));
-                const int n = ryr_get(w[j], a_) + ryr_get(w[j], a_ + 1);
-                int X = 0;
-                for (int i = 0; i < n; ++i)
-                    if (qdraw<PRNG, R>(q) < pr) X++;
-                w[j] = ryr_set(ryr_set(w[j], a_, (n > 0) ? X : 0), a_ + 1, (n > 0) ? n - X : 0);
-            }
-            ca[j] = tmp;
-        }
-        // update_CaJSR (SSA.hpp:67-71)
-        {
-            const R Jsum = Jrel[0] + Jrel[1] + Jrel[2] + Jrel[3];
-            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(q.jsr))));
-            q.jsr += static_cast<double>(h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
-        }
-        if (last) break;
-        if (out.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
-            out.err = 1;
-            break;
-        }
-        t += jump;
-        // sample_weights over the subunits (common.hpp:49-59)
-        int js;
-        {
-            const R u = qdraw<PRNG, R>(q) * total;
+        jump = ldiv(lneglog(qdraw<PRNG, R>(S.q)), total);
+        last = !(S.t + jump < dt);
+        h = last ? dt - S.t : jump;
+        if (!last) {
+            // sample_weights over the subunits (common.hpp:49-59)
+            const R u = qdraw<PRNG, R>(S.q) * total;
             R cum = s[0];
-            js = 0;
             if (cum < u) { js = 1; cum += s[1];
                 if (cum < u) { js = 2; cum += s[2];
                     if (cum < u) { js = 3; } } }
+            // the class and transition of sample_new_state (SSA.tpp:262-283) on subunit js
+            const uint64_t wj = sel4(S.w, js);
+            const R oldj = sel4(old, js);
+            const R sj = sel4(s, js);
+            const int L = sub_lcc(wj);
+            const R sum_LCC = T.P[L] + T.Q[L] * oldj;
+            const R inact = sub_y(wj) ? K.inact1 : K.inact0;
+            const R clr = sub_cl(wj) ? P.kbClCh : P.kfClCh * oldj;
+            const R sum_RyR = sj - (sum_LCC + inact + clr);
+            const R u3 = qdraw<PRNG, R>(S.q) * sj;
+            cls = (u3 < sum_LCC) ? 0 : (u3 < (sum_LCC + inact)) ? 1 : (u3 < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
+            if (cls == 0) {
+                // sample_LCC (SSA.tpp:103-197)
+                const R r0 = T.R0[L];
+                const R r1 = (L == 1) ? T.Q[L] * oldj : T.R1[L];
+                const R uu = qdraw<PRNG, R>(S.q) * sum_LCC;
+                R c = r0;
+                if (c < uu) { tr = 1; c += r1;
+                    if (c < uu) { tr = 2; } }
+            } else if (cls == 2) {
+                tr = ryr_pick(P, wj, oldj, qdraw<PRNG, R>(S.q) * sum_RyR);
+            }
         }
-        // sample_new_state (SSA.tpp:262-283) on subunit js
-        const R u3 = qdraw<PRNG, R>(q);
-        uint64_t wj = sel4(w, js);
-        const uint64_t wpj = sel4(w_pre, js);
-        const R oldj = sel4(old, js);
-        const R sj = sel4(s, js);
-        const R caj = static_cast<R>(sel4(ca, js));
-        R jlj = sel4(jl, js);
-        R Jrelj = sel4(Jrel, js);
-        int openj = sel4(open, js);
-        const int L = sub_lcc(wj);
-        const int y = sub_y(wj);
-        const int cl = sub_cl(wj);
-        const R sum_LCC = T.P[L] + T.Q[L] * oldj;
-        const R inact = y ? K.inact1 : K.inact0;
-        const R clr = cl ? P.kbClCh : P.kfClCh * oldj;
-        const R sum_RyR = sj - (sum_LCC + inact + clr);
-        const R u = u3 * sj;
-        const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
-        const R u4 = (cls == 0 || cls == 2) ? qdraw<PRNG, R>(q) : R(0);
-        if (cls == 0) {
-            // sample_LCC (SSA.tpp:103-197)
-            const R r0 = T.R0[L];
-            const R r1 = (L == 1) ? T.Q[L] * oldj : T.R1[L];
-            const R uu = u4 * sum_LCC;
-            R cum = r0;
-            int tr = 0;
-            if (cum < uu) { tr = 1; cum += r1;
-                if (cum < uu) { tr = 2; } }
-            const int Ln = lcc_next(L, tr);
-            if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
-                jlj = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj) : R(0);
-            else if (L == 6 || L == 12)
-                jlj = R(0);
-            wj = sub_set_lcc(wj, Ln);
-        } else if (cls == 1) {
-            const int yn = 1 - y;
-            wj = sub_flip_y(wj);
-            if (yn == 0) jlj = R(0);
-            else if (L == 6 || L == 12) jlj = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj);
-        } else if (cls == 2) {
-            // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
-            const R uu = u4 * sum_RyR;
-            int tr = 0;
-            R cum = R(0);
-            bool done = false;
-            ryr_rates(P, wpj, oldj, [&](int k, R r) {
-                if (done) return;
-                if (k == 0) { cum = r; }
-                else { if (cum < uu) { tr = k; cum += r; } else done = true; }
-            });
-            const int from = (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
-                           : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
-            const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
-                         : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
-            wj = ryr_add(ryr_add(wj, from, -1), to, +1);
+    }
+    // update_CaSS (SSA.tpp:87-99): subunits in order, re-splits drawn as they come
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const R Jiss = P.riss * static_cast<R>(oldA[(j + 1) & 3] + oldA[(j + 3) & 3] - 2 * oldA[j]);
+        S.Jxfer[j] = P.rxfer * static_cast<R>(oldA[j] - Cai);
+        const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old[j])) + ldiv(P.BSL_const, sq(P.KBSL + old[j])));
+        const R dCaSS = h * (S.jl[j] + S.Jrel[j] - S.Jxfer[j] + Jiss) * betaSS;
+        const double tmp = oldA[j] + static_cast<double>(dCaSS);
+        int split = split_mask<R>(oldA[j], tmp);
+        while (split) {
+            // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), the (5,6) pair first
+            const int bit = split & -split;
+            split ^= bit;
+            const int a_ = (bit == 1) ? 4 : 2;
+            const R pr = (bit == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(old[j])) : ldiv(P.k43, P.k43 + P.k34 * sq(old[j]));
+            const int n = ryr_get(S.w[j], a_) + ryr_get(S.w[j], a_ + 1);
+            int X = 0;
+            for (int i = 0; i < n; ++i)
+                if (qdraw<PRNG, R>(S.q) < pr) X++;
+            S.w[j] = ryr_set(ryr_set(S.w[j], a_, (n > 0) ? X : 0), a_ + 1, (n > 0) ? n - X : 0);
+        }
+        S.ca[j] = tmp;
+    }
+    // update_CaJSR (SSA.hpp:67-71)
+    {
+        const R Jsum = S.Jrel[0] + S.Jrel[1] + S.Jrel[2] + S.Jrel[3];
+        const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(S.q.jsr))));
+        S.q.jsr += static_cast<double>(h * betaJSR * (S.Jtr - P.VSS_VJSR * Jsum));
+    }
+    if (last) return true;
+    if (S.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
+        S.err = 1;
+        return true;
+    }
+    S.t += jump;
+    // apply the chosen event to subunit js after the update (SSA.tpp:262-283)
+    uint64_t wj = sel4(S.w, js);
+    const double caj_d = sel4(S.ca, js);
+    const R caj = static_cast<R>(caj_d);
+    R jlj = sel4(S.jl, js);
+    R Jrelj = sel4(S.Jrel, js);
+    int openj = sel4(S.open, js);
+    const int L = sub_lcc(wj);
+    const int y = sub_y(wj);
+    if (cls == 0) {
+        const int Ln = lcc_next(L, tr);
+        if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
+            jlj = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj) : R(0);
+        else if (L == 6 || L == 12)
+            jlj = R(0);
+        wj = sub_set_lcc(wj, Ln);
+    } else if (cls == 1) {
+        const int yn = 1 - y;
+        wj = sub_flip_y(wj);
+        if (yn == 0) jlj = R(0);
+        else if (L == 6 || L == 12) jlj = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caj);
+    } else if (cls == 2) {
+        // sample_RyR (SSA.tpp:200-259): the transition chosen above, unless it
+        // would take a RyR out of an empty state (ryr_allowed)
+        if (tr >= 0 && ryr_allowed(wj, tr, caj_d)) {
+            wj = ryr_apply(wj, tr);
             if (tr == 1 || tr == 10) openj += 1;
             else if (tr == 4 || tr == 7) openj -= 1;
-            Jrelj = P.rRyR * static_cast<R>(openj) * static_cast<R>(q.jsr - sel4(ca, js));
-        } else {
-            wj = sub_flip_cl(wj);
         }
-        put4(w, js, wj);
-        put4(jl, js, jlj);
-        put4(Jrel, js, Jrelj);
-        put4(open, js, openj);
+        Jrelj = P.rRyR * static_cast<R>(openj) * static_cast<R>(S.q.jsr - caj_d);
+    } else {
+        wj = sub_flip_cl(wj);
     }
-    // outputs of update_CRUstate_from_temp (GW.tpp:47-62), summed in subunit order in FP64
-    out.Jxfer = ((static_cast<double>(Jxfer[0]) + Jxfer[1]) + Jxfer[2]) + Jxfer[3];
-    out.Jtr = static_cast<double>(Jtr);
-    out.JLCC = ((static_cast<double>(jl[0]) + jl[1]) + jl[2]) + jl[3];
-    out.Jrel = ((static_cast<double>(Jrel[0]) + Jrel[1]) + Jrel[2]) + Jrel[3];
+    put4(S.w, js, wj);
+    put4(S.jl, js, jlj);
+    put4(S.Jrel, js, Jrelj);
+    put4(S.open, js, openj);
+    return false;
+}
+
+// outputs of update_CRUstate_from_temp (GW.tpp:47-62), summed in subunit order in FP64
+__device__ __forceinline__ LaneOut lane_outputs(const LaneState &S)
+{
+    LaneOut out;
+    out.Jxfer = ((static_cast<double>(S.Jxfer[0]) + S.Jxfer[1]) + S.Jxfer[2]) + S.Jxfer[3];
+    out.Jtr = static_cast<double>(S.Jtr);
+    out.JLCC = ((static_cast<double>(S.jl[0]) + S.jl[1]) + S.jl[2]) + S.jl[3];
+    out.Jrel = ((static_cast<double>(S.Jrel[0]) + S.Jrel[1]) + S.Jrel[2]) + S.Jrel[3];
     out.open = 0;
     out.clch = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        out.open += ryr_get(w[j], 2) + ryr_get(w[j], 3);
-        out.clch += sub_cl(w[j]);
+        out.open += ryr_get(S.w[j], 2) + ryr_get(S.w[j], 3);
+        out.clch += sub_cl(S.w[j]);
     }
+    out.iters = S.iters;
+    out.err = S.err;
     return out;
 }
 

@@ -286,29 +286,37 @@ int gw_batch_set_mode(gw_batch *b, int32_t frozen, int32_t digest);
 /* Total SSA loop iterations executed (diagnostic for the flop count per CRU-step). */
 int gw_batch_get_iterations(gw_batch *b, uint64_t *iters);
 /*
- * Launch shape of the SSA kernel: lanes per CRU (4: a quad of lanes per CRU,
- * one subunit per lane; 1: one lane per CRU, FP32-local batches only; 0 = 4)
- * and threads per block (4 lanes: 64, 128, 256 or 512; 1 lane: 64 or 128;
- * 0 = 256 or 64). FP64 results do not depend on it (exact flux sums). Shapes
- * other than 4 x 256 exist for xoshiro256** batches without replay digests or
- * frozen globals. No reference counterpart (the reference has one OpenMP
- * loop over CRUs, src/includes/GW.tpp:223-233).
+ * Launch shape of the SSA kernel. lanes_per_cru: 4 (a quad of lanes per CRU,
+ * one subunit per lane) or 1 (one lane per CRU, FP32-local batches only);
+ * 0 = 4. block: threads per block, 64, 128, 256 or 512 with 4 lanes, 64, 128
+ * or 256 with 1 lane; 0 = 256 or 64. crus_per_lane (1 lane only): 1, 2, 4 or
+ * 8; each wave then owns 64 x crus_per_lane consecutive CRUs and a lane whose
+ * CRU has finished its step takes the next one, so waves wait less on their
+ * slowest CRU; 0 = 1. FP64 results do not depend on the shape (exact flux
+ * sums). Shapes other than 4 x 256 exist for xoshiro256** batches without
+ * replay digests or frozen globals. No reference counterpart (the reference
+ * has one OpenMP loop over CRUs, src/includes/GW.tpp:223-233).
  */
-int gw_batch_set_launch(gw_batch *b, int32_t lanes_per_cru, int32_t block,
+#define GW_LAUNCH_CONFIGS 16
+/* The shape of launch configuration `index` (0 .. GW_LAUNCH_CONFIGS-1), the order of autotune's timings. */
+int gw_launch_config(int32_t index, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane);
+int gw_batch_set_launch(gw_batch *b, int32_t lanes_per_cru, int32_t block, int32_t crus_per_lane,
                         char *err, size_t errlen);
-int gw_batch_get_launch(gw_batch *b, int32_t *lanes_per_cru, int32_t *block);
+int gw_batch_get_launch(gw_batch *b, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane);
 /*
- * Autotuner: times trial_steps Euler steps of the batch in every available
- * launch shape, each trial starting from the same state (the batch state is
- * saved before and restored after), keeps the fastest, then times that shape
- * with its steps replayed as captured graphs of 64 steps and keeps graphs if
- * they are faster (*graph_out: 64 or 0). ms_per_step[7] (may be NULL)
- * receives the time per step of the shapes 4x64, 4x128, 4x256, 4x512, 1x64
- * and 1x128 (lanes per CRU x threads; -1 where not available) and of the
- * graph trial.
+ * Autotuner: times trial_steps Euler steps of the batch in every launch
+ * configuration available to it, each trial starting from the same state (the
+ * batch state is saved before and restored after), keeps the fastest, then
+ * times that shape with its steps replayed as captured graphs of 64 steps and
+ * keeps graphs if they are faster (*graph_out: 64 or 0). The trials run from
+ * the batch's current state: tune on a state like the one the run will spend
+ * its time in. ms_per_step[GW_LAUNCH_CONFIGS + 1] (may be NULL) receives the
+ * time per step of each configuration (gw_launch_config order; -1 where not
+ * available) and of the graph trial.
  */
 int gw_batch_autotune(gw_batch *b, int64_t trial_steps, int32_t *lanes_out, int32_t *block_out,
-                      int32_t *graph_out, double *ms_per_step, char *err, size_t errlen);
+                      int32_t *crus_per_lane_out, int32_t *graph_out, double *ms_per_step,
+                      char *err, size_t errlen);
 /*
  * Time nsteps Euler steps with HIP events on the batch's stream: *ms is the
  * wall time of the whole sequence, *launches the number of SSA dispatches.

@@ -165,7 +165,7 @@ def test_launch_shapes_give_identical_results(gwcapi, gpu_ctx, precision):
         b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, s, None, pace) for n, s in specs], precision=precision)
         b.set_mode(frozen=False, digest=False)
         b.set_launch(4, blk)
-        assert b.launch() == (4, blk)
+        assert b.launch() == (4, blk, 1)
         b.step(1500)
         snap = _snapshot(b, len(specs))
         b.close()
@@ -187,9 +187,10 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
     a.step(300)
     shape, ms = a.autotune(40)
     say(f"  chose {shape}: {ms}")
-    assert shape in ((4, 64), (4, 128), (4, 256), (4, 512)) and a.launch() == shape
+    assert shape in ((4, 64, 1), (4, 128, 1), (4, 256, 1), (4, 512, 1)) and a.launch() == shape
     assert all(ms[k] is not None and ms[k] > 0 for k in ("4x64", "4x128", "4x256", "4x512"))
-    assert ms["1x64"] is None and ms["1x128"] is None  # one lane per CRU is FP32-local only
+    lane = [k for k in ms if k.startswith("1x")]
+    assert len(lane) == 12 and all(ms[k] is None for k in lane)  # one lane per CRU is FP32-local only
     a.step(300)
     b = gwcapi.Batch(gpu_ctx, cells)
     b.set_mode(frozen=False, digest=False)
@@ -207,8 +208,9 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
     b.close()
 
 
-@pytest.mark.parametrize("precision,launch,digest", [(0, (4, 256), True), (0, (4, 64), False), (1, (1, 128), False)],
-                         ids=["fp64-digest", "fp64-4x64", "fp32-1x128"])
+@pytest.mark.parametrize("precision,launch,digest",
+                         [(0, (4, 256, 1), True), (0, (4, 64, 1), False), (1, (1, 128, 1), False), (1, (1, 64, 4), False)],
+                         ids=["fp64-digest", "fp64-4x64", "fp32-1x128", "fp32-1x64x4"])
 def test_graph_replays_match_plain_launches(gwcapi, gpu_ctx, precision, launch, digest):
     """Steps replayed as captured graphs give the same bits as plain launches,
     over calls of several lengths with the host reading the state between them."""
