@@ -98,28 +98,45 @@ def make_cells(capi, workload, cells, crus):
 
 def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200):
     ctx = capi.Context(0)
-    b = capi.Batch(ctx, cell_list, precision=precision)
-    b.set_mode(frozen=False, digest=False)
-    if warmup > 0:
-        b.step(warmup)
     tune = None
-    if autotune > 0:  # trials start from the warmed-up state, which is restored afterwards
-        (lanes, blk), trials = b.autotune(autotune)
-        tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_block": blk // lanes,
-                "graph_steps": b.graph_steps, "trial_steps": autotune, "trial_ms_per_step": trials}
+    if autotune > 0:
+        # Tune on the state the timed region spends its time in: a copy of the
+        # batch advanced to the middle of that region (the autotuner restores
+        # the state it starts from after its trials).
+        t = capi.Batch(ctx, cell_list, precision=precision)
+        t.set_mode(frozen=False, digest=False)
+        t.step(warmup + steps // 2)
+        (lanes, blk, cpl), trials = t.autotune(autotune)
+        tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_lane": cpl, "crus_per_block": blk * cpl // lanes,
+                "graph_steps": t.graph_steps, "trial_steps": autotune, "tuned_at_step": warmup + steps // 2,
+                "trial_ms_per_step": trials}
+        t.close()
+
+    def fresh():
+        x = capi.Batch(ctx, cell_list, precision=precision)
+        x.set_mode(frozen=False, digest=False)
+        if tune is not None:
+            x.set_launch(tune["lanes_per_cru"], tune["block"], tune["crus_per_lane"])
+            x.set_graph(tune["graph_steps"])
+        if warmup > 0:
+            x.step(warmup)
+        return x
+
+    # The timed region runs the steps as a caller would (plain launches or
+    # replayed graphs, no per-dispatch events, which would add a gap between
+    # launches). The step kernel's duration comes from a replay of the same
+    # steps (same cells and seeds: the run is deterministic) launched with
+    # per-dispatch events.
+    r = fresh()
+    _, k_ms, k_launches = r.time_steps(steps, per_kernel=True)
+    r.close()
+    b = fresh()
     i0 = b.iterations()
-    if tune and tune["graph_steps"] > 0:
-        # replayed graphs carry no per-dispatch events: the step kernel's duration
-        # comes from plain launches right after the timed region
-        ms, _, launches = b.time_steps(steps, per_kernel=False)
-        iters = b.iterations() - i0
-        k_steps = min(steps, 2000)
-        _, k_ms, k_launches = b.time_steps(k_steps, per_kernel=True)
-        ssa_ms = k_ms / k_launches * launches
-        tune["kernel_timing"] = f"plain launches of the {k_steps} steps after the timed region"
-    else:
-        ms, ssa_ms, launches = b.time_steps(steps, per_kernel=True)
-        iters = b.iterations() - i0
+    ms, _, launches = b.time_steps(steps, per_kernel=False)
+    iters = b.iterations() - i0
+    ssa_ms = k_ms / k_launches * launches
+    if tune is not None:
+        tune["kernel_timing"] = f"per-dispatch events over a replay of the {steps} timed steps"
     g = b.get_globals()
     obs = b.obs()
     b.close()
