@@ -96,10 +96,13 @@ def make_cells(capi, workload, cells, crus):
     return out, 500.0
 
 
-def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200):
+def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=None):
     ctx = capi.Context(0)
     tune = None
-    if autotune > 0:
+    if launch is not None:   # a fixed launch shape instead of the autotuner's choice
+        tune = {"lanes_per_cru": launch[0], "block": launch[1], "crus_per_lane": launch[2],
+                "crus_per_block": launch[1] * launch[2] // launch[0], "graph_steps": 0, "fixed": True}
+    elif autotune > 0:
         # Tune on the state the timed region spends its time in: a copy of the
         # batch advanced to the middle of that region (the autotuner restores
         # the state it starts from after its trials).
@@ -123,25 +126,32 @@ def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200):
         return x
 
     # The timed region runs the steps as a caller would (plain launches or
-    # replayed graphs, no per-dispatch events, which would add a gap between
-    # launches). The step kernel's duration comes from a replay of the same
-    # steps (same cells and seeds: the run is deterministic) launched with
-    # per-dispatch events.
-    r = fresh()
-    _, k_ms, k_launches = r.time_steps(steps, per_kernel=True)
-    r.close()
+    # replayed graphs) between two HIP events on the batch's stream. The step
+    # launches run back to back (no host gap: the kernel trace of the same
+    # command shows them so), so the region's time over its SSA launches is the
+    # step kernel's average duration. Per-dispatch timing events are not used:
+    # each makes its kernel's completion write back the caches, which adds
+    # about 3 us to every kernel.
     b = fresh()
     i0 = b.iterations()
     ms, _, launches = b.time_steps(steps, per_kernel=False)
     iters = b.iterations() - i0
-    ssa_ms = k_ms / k_launches * launches
+    ssa_ms = ms
     if tune is not None:
-        tune["kernel_timing"] = f"per-dispatch events over a replay of the {steps} timed steps"
+        tune["kernel_timing"] = f"HIP events around the {steps} timed steps ({launches} SSA launches and one close-only launch)"
     g = b.get_globals()
     obs = b.obs()
     b.close()
     ctx.close()
     return ms, ssa_ms, launches, iters, g, obs, tune
+
+
+def kernel_name(tune, f32):
+    """The step kernel a launch shape runs (xoshiro256** instantiation)."""
+    lanes, blk, cpl = (tune["lanes_per_cru"], tune["block"], tune["crus_per_lane"]) if tune else (4, 256, 1)
+    if lanes == 1:
+        return f"k_step_lane<0, {blk}, {cpl}>"
+    return f"k_step<0, false, {'float' if f32 else 'double'}, {blk}>"
 
 
 def measured_traffic(total_crus, form):
@@ -206,8 +216,10 @@ def measure(capi, precision, args, total_crus):
     """One form (FP64 or FP32-local) on the bench batch: throughput and the SSA kernel's roofline."""
     f32 = precision == capi.PREC_FP32_LOCAL
     cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
+    fixed = args.launch_fp32 if f32 else args.launch_fp64
+    launch = tuple(int(x) for x in (fixed + "x1").split("x")[:3]) if fixed else None
     ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, cell_list, args.steps, args.warmup,
-                                                          precision=precision, autotune=args.autotune)
+                                                          precision=precision, autotune=args.autotune, launch=launch)
     sim_ms = args.steps * 1e-3
     sec = ms / 1e3
     per_launch_s = ssa_ms / 1e3 / launches
@@ -233,9 +245,8 @@ def measure(capi, precision, args, total_crus):
             "unit": "TFLOP/s" if valu else "GB/s",
             "frac": max(frac_f, frac_b),
             "traffic": measured_traffic(total_crus, "fp32" if f32 else "fp64"),
-            "kernel": f"k_step<0,false,{'float' if f32 else 'double'}>",
+            "kernel": kernel_name(tune, f32),
             "avg_launch_us": per_launch_s * 1e6,
-            "ssa_share_of_step": (ssa_ms / ms) if ms > 0 else None,
             "valu": {"achieved_tflops": achieved_tflops, "peak": peak_f, "frac": frac_f,
                      "flops_per_launch": flops_per_launch, "f_iter": F_ITER},
             "hbm": {"achieved_gbs": achieved_gbs, "peak": PEAK_HBM_GBS, "frac": frac_b,
@@ -258,6 +269,9 @@ def main(argv=None):
                     help="seeds64 (default): the seed ensemble; batch256 / large1024: the uneven and the large batch")
     ap.add_argument("--autotune", type=int, default=200, metavar="STEPS",
                     help="autotuner trial steps per SSA block size before the timed region (0: default shape)")
+    ap.add_argument("--launch-fp32", default=None, metavar="LxB[xC]",
+                    help="fixed FP32-local launch shape (lanes per CRU x threads x CRUs per lane) instead of the autotuner")
+    ap.add_argument("--launch-fp64", default=None, metavar="LxB", help="fixed FP64 launch shape instead of the autotuner")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -1999,7 +1999,8 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
                 if ((he = hipEventElapsedTime(&f, e0, e1)) == hipSuccess) {
                     const double ms = f / static_cast<double>(trial_steps);
                     if (ms_per_step) ms_per_step[kNumConfigs] = ms;
-                    if (ms < best_ms) best_graph = kAutotuneGraphSteps;
+                    // graphs only when they save more than noise (launch-bound batches)
+                    if (ms < 0.98 * best_ms) best_graph = kAutotuneGraphSteps;
                 }
             }
         }
