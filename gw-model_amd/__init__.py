@@ -15,5 +15,6 @@ from . import capi  # noqa: E402,F401
 from .utils import *  # noqa: E402,F401,F403
 from .parameters import GWParameters  # noqa: E402,F401
 from .model import ICaL, Ito2, GWModel, GWSolution  # noqa: E402,F401
+from .serialization import sim_dict_reduced_memory  # noqa: E402,F401
 from .utils import assert_positive, assert_nonnegative, assert_type  # noqa: E402,F401
 import build.GreensteinWinslow as gw  # noqa: E402,F401

@@ -77,6 +77,16 @@ class Record(C.Structure):
         "CaJSR", "CaSS", "LCC", "LCC_inactivation", "ClCh", "RyR")]
 
 
+class CruStateReduced(C.Structure):
+    _fields_ = [(n, P) for n in ("CaSS", "CaJSR", "LCC", "LCC_inactivation", "ClCh", "RyR")]
+
+
+class RecordReduced(C.Structure):
+    _fields_ = [("num_records", i64)] + [(n, P) for n in (
+        "t", "V", "m", "h", "j", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN", "CaHTRPN", "xKs", "XKr", "XKv14", "XKv43",
+        "CaJSR", "CaSS", "LCC", "LCC_inactivation", "ClCh", "RyR")]
+
+
 class Protocol(C.Structure):
     _fields_ = [("kind", i32), ("_pad", i32), ("period", f64), ("duration", f64), ("amplitude", f64), ("start", f64),
                 ("hold", f64), ("step", f64), ("t_on", f64), ("t_off", f64)]
@@ -124,6 +134,9 @@ def lib() -> C.CDLL:
         "gw_batch_destroy": (C.c_int, [P]),
         "gw_batch_set_cru_state": (C.c_int, [P, i32, C.POINTER(CruState)] + E),
         "gw_batch_get_cru_state": (C.c_int, [P, i32, C.POINTER(CruState)] + E),
+        "gw_batch_get_cru_state_reduced": (C.c_int, [P, i32, C.POINTER(CruStateReduced)] + E),
+        "gw_run_reduced": (C.c_int, [P, C.POINTER(Params), i32, f64, i64, P, i32, C.POINTER(Globals),
+                                     C.POINTER(CruState), u64, i32, i32, C.POINTER(RecordReduced)] + E),
         "gw_batch_set_globals": (C.c_int, [P, i32, C.POINTER(Globals)] + E),
         "gw_batch_get_globals": (C.c_int, [P, P] + E),
         "gw_batch_set_stimulus_table": (C.c_int, [P, i32, P, i64] + E),
@@ -275,6 +288,16 @@ class Batch:
         _call(self.L.gw_batch_get_cru_state, self.h, cell, C.byref(s))
         return out
 
+    def get_cru_state_reduced(self, cell: int) -> dict:
+        """One cell's CRU state in the reduced record encoding (gw_cru_state_reduced_t)."""
+        n = self.cells[cell].n_cru
+        out = {"CaSS": np.zeros((n, 4), np.float32), "CaJSR": np.zeros(n, np.float32),
+               "LCC": np.zeros((n, 4), np.int8), "LCC_inactivation": np.zeros((n, 4), np.bool_),
+               "ClCh": np.zeros((n, 4), np.bool_), "RyR": np.zeros((n, 4), np.int16)}
+        s = CruStateReduced(*[_ptr(out[k]) for k in ("CaSS", "CaJSR", "LCC", "LCC_inactivation", "ClCh", "RyR")])
+        _call(self.L.gw_batch_get_cru_state_reduced, self.h, cell, C.byref(s))
+        return out
+
     def set_globals(self, cell: int, g: Globals):
         _call(self.L.gw_batch_set_globals, self.h, cell, C.byref(g))
 
@@ -363,6 +386,15 @@ class Batch:
         return ms.value, (ssa.value if per_kernel else None), launches.value
 
 
+def _crus_struct(crus: dict | None):
+    if crus is None:
+        return None, None
+    keep = [np.ascontiguousarray(crus["CaSS"], np.float64), np.ascontiguousarray(crus["CaJSR"], np.float64),
+            np.ascontiguousarray(crus["LCC"], np.int32), np.ascontiguousarray(crus["LCC_inactivation"], np.int32),
+            np.ascontiguousarray(crus["RyR"], np.int32), np.ascontiguousarray(crus["ClCh"], np.int32)]
+    return CruState(*[_ptr(x) for x in keep], None), keep
+
+
 def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, istim, record_every: int,
         globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0, precision: int = PREC_FP64) -> dict:
     """gw_run: the run<PRNG> loop (src/pyGW.hpp:102-122) with ceil(num_steps / record_every) records."""
@@ -379,14 +411,36 @@ def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, ist
     for k, v in out.items():
         setattr(rec, k, v.ctypes.data)
     ist = None if istim is None else np.ascontiguousarray(istim, np.float64)
-    keep = None
-    cs = None
-    if crus is not None:
-        keep = [np.ascontiguousarray(crus["CaSS"], np.float64), np.ascontiguousarray(crus["CaJSR"], np.float64),
-                np.ascontiguousarray(crus["LCC"], np.int32), np.ascontiguousarray(crus["LCC_inactivation"], np.int32),
-                np.ascontiguousarray(crus["RyR"], np.int32), np.ascontiguousarray(crus["ClCh"], np.int32)]
-        cs = CruState(*[_ptr(x) for x in keep], None)
+    cs, keep = _crus_struct(crus)
     _call(lib().gw_run, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
           C.byref(globals_) if globals_ is not None else None, C.byref(cs) if cs is not None else None,
           int(seed) & (2**64 - 1), prng, precision, C.byref(rec))
+    del keep
+    return out
+
+
+def run_reduced(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, istim, record_every: int,
+                globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0,
+                precision: int = PREC_FP64) -> dict:
+    """gw_run_reduced: run() with its records in the reduced encoding of
+    sim_dict_reduced_memory (src/serialization.py:11-39), formed on the device."""
+    R = -(-int(num_steps) // int(record_every))
+    n = int(n_cru)
+    out = {k: np.zeros(R, np.float32) for k in ("t", "V", "m", "h", "j", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN",
+                                                 "CaHTRPN", "xKs")}
+    out.update({"XKr": np.zeros((R, 4), np.float32), "XKv14": np.zeros((R, 9), np.float32),
+                "XKv43": np.zeros((R, 9), np.float32), "CaJSR": np.zeros((R, n), np.float32),
+                "CaSS": np.zeros((R, n, 4), np.float32), "LCC": np.zeros((R, n, 4), np.int8),
+                "LCC_inactivation": np.zeros((R, n, 4), np.bool_), "ClCh": np.zeros((R, n, 4), np.bool_),
+                "RyR": np.zeros((R, n, 4), np.int16)})
+    rec = RecordReduced()
+    rec.num_records = R
+    for k, v in out.items():
+        setattr(rec, k, v.ctypes.data)
+    ist = None if istim is None else np.ascontiguousarray(istim, np.float64)
+    cs, keep = _crus_struct(crus)
+    _call(lib().gw_run_reduced, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
+          C.byref(globals_) if globals_ is not None else None, C.byref(cs) if cs is not None else None,
+          int(seed) & (2**64 - 1), prng, precision, C.byref(rec))
+    del keep
     return out

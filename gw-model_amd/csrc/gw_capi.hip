@@ -876,6 +876,42 @@ __global__ void k_get_state(KArgs a, int cell, StateView v)
     v.CaJSR[i] = ld_cajsr(a, slot);
 }
 
+// One cell's CRU state in the reduced record encoding (gw_cru_state_reduced_t).
+struct ReducedView {
+    float *CaSS, *CaJSR;
+    int16_t *RyR;
+    int8_t *LCC;
+    uint8_t *LCCi, *ClCh;
+};
+
+// RyR_state_to_int16 (src/utils.py:8-22) of one subunit word: a lumped pair
+// with a negative member counts as its sum in the pair's first member.
+__device__ __forceinline__ int16_t ryr_code(uint64_t w)
+{
+    const int r1 = ryr_get(w, 0), r2 = ryr_get(w, 1), r3 = ryr_get(w, 2), r4 = ryr_get(w, 3);
+    const int r5 = ryr_get(w, 4), r6 = ryr_get(w, 5);
+    const bool neg34 = r3 < 0 || r4 < 0, neg56 = r5 < 0 || r6 < 0;
+    const int s3 = neg34 ? r3 + r4 : r3, s4 = neg34 ? 0 : r4, s5 = neg56 ? r5 + r6 : r5;
+    return static_cast<int16_t>(r1 + 6 * r2 + 36 * s3 + 216 * s4 + 1296 * s5);
+}
+
+__global__ void k_get_state_reduced(KArgs a, int cell, ReducedView v)
+{
+    const DevCell &C = a.cells[cell];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= C.n_cru) return;
+    const int64_t slot = C.slot0 + i;
+    for (int j = 0; j < 4; ++j) {
+        v.CaSS[i * 4 + j] = static_cast<float>(ld_cass(a, slot * 4 + j));
+        const uint64_t w = a.sub[slot * 4 + j];
+        v.LCC[i * 4 + j] = static_cast<int8_t>(sub_lcc(w));
+        v.LCCi[i * 4 + j] = static_cast<uint8_t>(sub_y(w));
+        v.ClCh[i * 4 + j] = static_cast<uint8_t>(sub_cl(w));
+        v.RyR[i * 4 + j] = ryr_code(w);
+    }
+    v.CaJSR[i] = static_cast<float>(ld_cajsr(a, slot));
+}
+
 __global__ void k_reset_obs(CellDyn *dyn, int ncells)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1290,7 +1326,7 @@ int check_status(gw_batch *B, char *err, size_t errlen)
         if (st != 0) {
             const char *what = st == GW_E_NONPOSITIVE_CONC ? "a Nernst potential received a non-positive concentration"
                              : st == GW_E_NAN ? "a global state variable became NaN"
-                             : st == GW_E_RANGE ? "a CRU flux or the SSA iteration count left the supported range"
+                             : st == GW_E_RANGE ? "a CRU flux, a RyR count or the SSA iteration count left the supported range"
                                                 : "device error";
             set_err(err, errlen, "cell %d: %s (status %d)", c, what, st);
             return st;
@@ -1326,6 +1362,20 @@ int ensure_staging(gw_batch *B, size_t bytes, char *err, size_t errlen)
 
 // staging layout for one cell of n CRUs: CaSS[4n] CaJSR[n] JLCC[4n] (f64), LCC LCCi ClCh [4n] RyR[24n] (i32)
 size_t staging_size(int64_t n) { return static_cast<size_t>(n) * (9 * 8 + 36 * 4); }
+
+// The reduced view in the same staging buffer (40 bytes per CRU).
+ReducedView reduced_view(void *base, int64_t n)
+{
+    ReducedView v;
+    char *p = static_cast<char *>(base);
+    v.CaSS = reinterpret_cast<float *>(p); p += 4 * n * 4;
+    v.CaJSR = reinterpret_cast<float *>(p); p += n * 4;
+    v.RyR = reinterpret_cast<int16_t *>(p); p += 4 * n * 2;
+    v.LCC = reinterpret_cast<int8_t *>(p); p += 4 * n;
+    v.LCCi = reinterpret_cast<uint8_t *>(p); p += 4 * n;
+    v.ClCh = reinterpret_cast<uint8_t *>(p);
+    return v;
+}
 
 StateView staging_view(void *base, int64_t n)
 {
@@ -1664,6 +1714,30 @@ int gw_batch_get_cru_state(gw_batch *B, int32_t cell, gw_cru_state_t *c, char *e
     if (c->LCC_inactivation) HIP_TRY(hipMemcpyAsync(c->LCC_inactivation, v.LCCi, 4 * n * 4, hipMemcpyDeviceToHost, s));
     if (c->ClCh) HIP_TRY(hipMemcpyAsync(c->ClCh, v.ClCh, 4 * n * 4, hipMemcpyDeviceToHost, s));
     if (c->RyR) HIP_TRY(hipMemcpyAsync(c->RyR, v.RyR, 24 * n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GW_OK;
+}
+
+int gw_batch_get_cru_state_reduced(gw_batch *B, int32_t cell, gw_cru_state_reduced_t *c, char *err, size_t errlen)
+{
+    if (!B || !c || cell < 0 || cell >= B->ncells) {
+        set_err(err, errlen, "gw_batch_get_cru_state_reduced: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    const int64_t n = B->cells_h[cell].n_cru;
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = ensure_staging(B, staging_size(n), err, errlen);
+    if (rc) return rc;
+    const ReducedView v = reduced_view(B->d_staging, n);
+    hipStream_t s = B->ctx->stream;
+    hipLaunchKernelGGL(k_get_state_reduced, dim3((n + 255) / 256), dim3(256), 0, s, B->a, cell, v);
+    HIP_TRY(hipGetLastError());
+    if (c->CaSS) HIP_TRY(hipMemcpyAsync(c->CaSS, v.CaSS, 4 * n * 4, hipMemcpyDeviceToHost, s));
+    if (c->CaJSR) HIP_TRY(hipMemcpyAsync(c->CaJSR, v.CaJSR, n * 4, hipMemcpyDeviceToHost, s));
+    if (c->RyR) HIP_TRY(hipMemcpyAsync(c->RyR, v.RyR, 4 * n * 2, hipMemcpyDeviceToHost, s));
+    if (c->LCC) HIP_TRY(hipMemcpyAsync(c->LCC, v.LCC, 4 * n, hipMemcpyDeviceToHost, s));
+    if (c->LCC_inactivation) HIP_TRY(hipMemcpyAsync(c->LCC_inactivation, v.LCCi, 4 * n, hipMemcpyDeviceToHost, s));
+    if (c->ClCh) HIP_TRY(hipMemcpyAsync(c->ClCh, v.ClCh, 4 * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return GW_OK;
 }
@@ -2078,9 +2152,17 @@ int gw_batch_set_mode(gw_batch *B, int32_t frozen, int32_t digest)
     return GW_OK;
 }
 
-int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
-           int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
-           int32_t precision, gw_record_t *out, char *err, size_t errlen)
+}  // extern "C"
+
+namespace {
+
+// The loop of run<PRNG> (src/pyGW.hpp:102-122) on a one-cell batch; record(B,
+// k, t) writes record k (after step i with i % record_every == 0) while k is
+// below num_records.
+template <class Record>
+int run_loop(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
+             int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
+             int32_t precision, int64_t num_records, Record &&record, char *err, size_t errlen)
 {
     if (!ctx || !p || n_cru <= 0 || num_steps < 0 || record_every <= 0) {
         set_err(err, errlen, "gw_run: invalid arguments (n_cru > 0, num_steps >= 0, record_every > 0 required)");
@@ -2106,10 +2188,8 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t 
         rc = gw_batch_set_stimulus_table(B, 0, istim, num_steps, err, errlen);
         if (rc) { gw_batch_destroy(B); return rc; }
     }
-    const int64_t n = n_cru;
     double t = 0.0;
     int64_t counter = 0;
-    std::vector<double> dummyd;
     int64_t i = 0;
     while (i < num_steps) {
         // next record after step i_rec (i_rec % record_every == 0, i_rec >= i)
@@ -2119,28 +2199,9 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t 
         if (rc) { gw_batch_destroy(B); return rc; }
         for (int64_t k = i; k < upto; ++k) t += dt;
         i = upto;
-        if (out && i == i_rec + 1) {
-            if (counter < out->num_records) {
-                gw_globals_t g;
-                rc = gw_batch_get_globals(B, &g, err, errlen);
-                if (rc) { gw_batch_destroy(B); return rc; }
-                const int64_t k = counter;
-#define PUT(f, v) if (out->f) out->f[k] = (v)
-                PUT(t, t); PUT(V, g.V); PUT(m, g.m); PUT(h, g.h); PUT(j, g.j); PUT(Nai, g.Nai); PUT(Ki, g.Ki);
-                PUT(Cai, g.Cai); PUT(CaNSR, g.CaNSR); PUT(CaLTRPN, g.CaLTRPN); PUT(CaHTRPN, g.CaHTRPN); PUT(xKs, g.xKs);
-#undef PUT
-                if (out->XKr) for (int q = 0; q < 5; ++q) out->XKr[k * 5 + q] = g.Kr[q];
-                if (out->XKv14) for (int q = 0; q < 10; ++q) out->XKv14[k * 10 + q] = g.Kv14[q];
-                if (out->XKv43) for (int q = 0; q < 10; ++q) out->XKv43[k * 10 + q] = g.Kv43[q];
-                gw_cru_state_t c;
-                c.CaSS = out->CaSS ? out->CaSS + k * n * 4 : nullptr;
-                c.CaJSR = out->CaJSR ? out->CaJSR + k * n : nullptr;
-                c.LCC = out->LCC ? out->LCC + k * n * 4 : nullptr;
-                c.LCC_inactivation = out->LCC_inactivation ? out->LCC_inactivation + k * n * 4 : nullptr;
-                c.ClCh = out->ClCh ? out->ClCh + k * n * 4 : nullptr;
-                c.RyR = out->RyR ? out->RyR + k * n * 24 : nullptr;
-                c.JLCC = nullptr;
-                rc = gw_batch_get_cru_state(B, 0, &c, err, errlen);
+        if (i == i_rec + 1) {
+            if (counter < num_records) {
+                rc = record(B, counter, t);
                 if (rc) { gw_batch_destroy(B); return rc; }
             }
             ++counter;
@@ -2148,6 +2209,70 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t 
     }
     gw_batch_destroy(B);
     return GW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
+           int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
+           int32_t precision, gw_record_t *out, char *err, size_t errlen)
+{
+    const int64_t n = n_cru;
+    auto record = [&](gw_batch *B, int64_t k, double t) {
+        gw_globals_t g;
+        int rc = gw_batch_get_globals(B, &g, err, errlen);
+        if (rc) return rc;
+#define PUT(f, v) if (out->f) out->f[k] = (v)
+        PUT(t, t); PUT(V, g.V); PUT(m, g.m); PUT(h, g.h); PUT(j, g.j); PUT(Nai, g.Nai); PUT(Ki, g.Ki);
+        PUT(Cai, g.Cai); PUT(CaNSR, g.CaNSR); PUT(CaLTRPN, g.CaLTRPN); PUT(CaHTRPN, g.CaHTRPN); PUT(xKs, g.xKs);
+#undef PUT
+        if (out->XKr) for (int q = 0; q < 5; ++q) out->XKr[k * 5 + q] = g.Kr[q];
+        if (out->XKv14) for (int q = 0; q < 10; ++q) out->XKv14[k * 10 + q] = g.Kv14[q];
+        if (out->XKv43) for (int q = 0; q < 10; ++q) out->XKv43[k * 10 + q] = g.Kv43[q];
+        gw_cru_state_t c;
+        c.CaSS = out->CaSS ? out->CaSS + k * n * 4 : nullptr;
+        c.CaJSR = out->CaJSR ? out->CaJSR + k * n : nullptr;
+        c.LCC = out->LCC ? out->LCC + k * n * 4 : nullptr;
+        c.LCC_inactivation = out->LCC_inactivation ? out->LCC_inactivation + k * n * 4 : nullptr;
+        c.ClCh = out->ClCh ? out->ClCh + k * n * 4 : nullptr;
+        c.RyR = out->RyR ? out->RyR + k * n * 24 : nullptr;
+        c.JLCC = nullptr;
+        return gw_batch_get_cru_state(B, 0, &c, err, errlen);
+    };
+    return run_loop(ctx, p, n_cru, dt, num_steps, istim, record_every, g0, c0, seed, prng_id, precision,
+                    out ? out->num_records : 0, record, err, errlen);
+}
+
+int gw_run_reduced(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps,
+                   const double *istim, int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0,
+                   uint64_t seed, int32_t prng_id, int32_t precision, gw_record_reduced_t *out, char *err,
+                   size_t errlen)
+{
+    const int64_t n = n_cru;
+    auto record = [&](gw_batch *B, int64_t k, double t) {
+        gw_globals_t g;
+        int rc = gw_batch_get_globals(B, &g, err, errlen);
+        if (rc) return rc;
+#define PUT(f, v) if (out->f) out->f[k] = static_cast<float>(v)
+        PUT(t, t); PUT(V, g.V); PUT(m, g.m); PUT(h, g.h); PUT(j, g.j); PUT(Nai, g.Nai); PUT(Ki, g.Ki);
+        PUT(Cai, g.Cai); PUT(CaNSR, g.CaNSR); PUT(CaLTRPN, g.CaLTRPN); PUT(CaHTRPN, g.CaHTRPN); PUT(xKs, g.xKs);
+#undef PUT
+        if (out->XKr) for (int q = 0; q < 4; ++q) out->XKr[k * 4 + q] = static_cast<float>(g.Kr[q]);
+        if (out->XKv14) for (int q = 0; q < 9; ++q) out->XKv14[k * 9 + q] = static_cast<float>(g.Kv14[q]);
+        if (out->XKv43) for (int q = 0; q < 9; ++q) out->XKv43[k * 9 + q] = static_cast<float>(g.Kv43[q]);
+        gw_cru_state_reduced_t c;
+        c.CaSS = out->CaSS ? out->CaSS + k * n * 4 : nullptr;
+        c.CaJSR = out->CaJSR ? out->CaJSR + k * n : nullptr;
+        c.LCC = out->LCC ? out->LCC + k * n * 4 : nullptr;
+        c.LCC_inactivation = out->LCC_inactivation ? out->LCC_inactivation + k * n * 4 : nullptr;
+        c.ClCh = out->ClCh ? out->ClCh + k * n * 4 : nullptr;
+        c.RyR = out->RyR ? out->RyR + k * n * 4 : nullptr;
+        return gw_batch_get_cru_state_reduced(B, 0, &c, err, errlen);
+    };
+    return run_loop(ctx, p, n_cru, dt, num_steps, istim, record_every, g0, c0, seed, prng_id, precision,
+                    out ? out->num_records : 0, record, err, errlen);
 }
 
 int gw_run_batch(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, double dt, int64_t nsteps,

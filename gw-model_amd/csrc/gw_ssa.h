@@ -298,11 +298,17 @@ __device__ __forceinline__ int ryr_pick(const SsaParamsT<R> &P, uint64_t w, R ca
     return tr;
 }
 
-// Source state (0..5) of RyR transition tr (0..11, in sample_RyR's order).
+// Source and target states (0..5) of RyR transition tr (0..11, in
+// sample_RyR's order, SSA.tpp:200-259).
 GW_HD int ryr_from(int tr)
 {
     return (tr == 0) ? 0 : (tr <= 2 || tr == 6) ? 1 : (tr == 3 || tr == 7) ? 2
          : (tr == 4 || tr == 8) ? 3 : (tr == 5 || tr == 9 || tr == 10) ? 4 : 5;
+}
+GW_HD int ryr_to(int tr)
+{
+    return (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
+         : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
 }
 
 // FP32-local only: whether transition tr may be applied to word w at the
@@ -335,13 +341,20 @@ GW_HD int split_mask(double oldA, double tmp)
     return m56 | m34;
 }
 
+// Whether transition tr keeps both counts it changes within the signed byte
+// a subunit word holds them in. Lumped-pair members drift (a transition at a
+// pair-sum rate moves one named member), and nothing bounds the drift; the
+// reference counts in int, so a count leaving [-128, 127] is reported
+// (GW_E_RANGE) instead of wrapping.
+GW_HD bool ryr_in_range(uint64_t w, int tr)
+{
+    return ryr_get(w, ryr_from(tr)) > -128 && ryr_get(w, ryr_to(tr)) < 127;
+}
+
 // Apply RyR transition tr (0..11, in sample_RyR's order) to a subunit word.
 GW_HD uint64_t ryr_apply(uint64_t w, int tr)
 {
-    const int from = ryr_from(tr);
-    const int to = (tr == 0 || tr == 7 || tr == 9) ? 1 : (tr == 1 || tr == 8) ? 2 : (tr == 3 || tr == 10) ? 3
-                 : (tr == 2 || tr == 4 || tr == 11) ? 4 : (tr == 5) ? 5 : 0;
-    return ryr_add(ryr_add(w, from, -1), to, +1);
+    return ryr_add(ryr_add(w, ryr_from(tr), -1), ryr_to(tr), +1);
 }
 
 struct CruQ {
@@ -506,7 +519,10 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
                 // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
                 int tr = ryr_pick(P, w_pre, old, u4 * sum_RyR);
                 if (sizeof(R) == 4 && tr >= 0 && !ryr_allowed(w, tr, ca)) tr = -1;
-                if (tr >= 0) w = ryr_apply(w, tr);
+                if (tr >= 0) {
+                    if (!ryr_in_range(w, tr)) out.err = 1;
+                    w = ryr_apply(w, tr);
+                }
                 if (tr == 1 || tr == 10) open += 1;
                 else if (tr == 4 || tr == 7) open -= 1;
                 Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
@@ -707,6 +723,7 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
         // sample_RyR (SSA.tpp:200-259): the transition chosen above, unless it
         // would take a RyR out of an empty state (ryr_allowed)
         if (tr >= 0 && ryr_allowed(wj, tr, caj_d)) {
+            if (!ryr_in_range(wj, tr)) S.err = 1;
             wj = ryr_apply(wj, tr);
             if (tr == 1 || tr == 10) openj += 1;
             else if (tr == 4 || tr == 7) openj -= 1;

@@ -14,6 +14,9 @@
 //   - The record buffer has ceil(num_steps / record_every) rows (the reference
 //     writes past num_steps / record_every rows when they do not divide).
 //   - A non-positive Nernst argument raises RuntimeError instead of exit().
+//   - run_reduced() (additive): run() with the records in the reduced encoding
+//     of sim_dict_reduced_memory (src/serialization.py:11-39), formed on the
+//     device.
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -169,35 +172,61 @@ int prng_id_from_name(const std::string &rng)
     return id;
 }
 
-// run_PRNG_arg + run<PRNG> (src/pyGW.cpp:134-157, src/pyGW.hpp:102-122)
-PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int num_steps, const py::object &Istim,
-                   int record_every, PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng,
-                   py::object seed)
+// The arguments of run() as the C-ABI takes them.
+struct RunInput {
+    int prng;
+    uint64_t seed;
+    std::vector<double> istim;
+    int64_t records;
+    gw_globals_t g0;
+    Arr<double> CaSS, CaJSR;
+    Arr<int32_t> LCC, LCCi, RyR, ClCh;
+    gw_cru_state_t c0;
+};
+
+RunInput run_input(int nCRU, double step_size, int num_steps, const py::object &Istim, int record_every,
+                   PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng, py::object seed)
 {
-    const int prng = prng_id_from_name(rng);
+    RunInput in;
+    in.prng = prng_id_from_name(rng);
     if (nCRU <= 0) throw py::value_error("nCRU must be > 0");
     if (record_every <= 0) throw py::value_error("record_every must be > 0");
     if (num_steps < 0) throw py::value_error("num_steps must be >= 0");
     if (init_crus.CaSS.shape(0) != nCRU || init_crus.CaJSR.shape(0) != nCRU || init_crus.LCC.shape(0) != nCRU ||
         init_crus.LCC_inactivation.shape(0) != nCRU || init_crus.RyR.shape(0) != nCRU || init_crus.ClCh.shape(0) != nCRU)
         throw py::value_error("init_crus must hold nCRU CRUs");
-    uint64_t s;
     if (seed.is_none()) {
         std::random_device rd;
-        s = (static_cast<uint64_t>(rd()) << 32) ^ rd();
+        in.seed = (static_cast<uint64_t>(rd()) << 32) ^ rd();
     } else {
-        s = seed.cast<uint64_t>();
+        in.seed = seed.cast<uint64_t>();
     }
     // Istim is evaluated at the accumulated time before every step (src/pyGW.hpp:110-116)
-    std::vector<double> istim(static_cast<size_t>(num_steps));
+    in.istim.resize(static_cast<size_t>(num_steps));
     {
         double t = 0.0;
         for (int i = 0; i < num_steps; ++i) {
-            istim[i] = Istim(t).cast<double>();
+            in.istim[i] = Istim(t).cast<double>();
             t += step_size;
         }
     }
-    const int R = num_steps == 0 ? 0 : (num_steps + record_every - 1) / record_every;
+    in.records = num_steps == 0 ? 0 : (static_cast<int64_t>(num_steps) + record_every - 1) / record_every;
+    in.g0 = init_globals.to_c();
+    in.CaSS = init_crus.CaSS; in.CaJSR = init_crus.CaJSR; in.LCC = init_crus.LCC;
+    in.LCCi = init_crus.LCC_inactivation; in.RyR = init_crus.RyR; in.ClCh = init_crus.ClCh;
+    in.c0 = gw_cru_state_t{const_cast<double *>(in.CaSS.data()), const_cast<double *>(in.CaJSR.data()),
+                           const_cast<int32_t *>(in.LCC.data()), const_cast<int32_t *>(in.LCCi.data()),
+                           const_cast<int32_t *>(in.RyR.data()), const_cast<int32_t *>(in.ClCh.data()), nullptr};
+    return in;
+}
+
+// run_PRNG_arg + run<PRNG> (src/pyGW.cpp:134-157, src/pyGW.hpp:102-122)
+PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int num_steps, const py::object &Istim,
+                   int record_every, PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng,
+                   py::object seed)
+{
+    RunInput in = run_input(nCRU, step_size, num_steps, Istim, record_every, init_crus, init_globals, rng, seed);
+    const int R = static_cast<int>(in.records);
     PyGWSimulation out(nCRU, R, num_steps * step_size);
     gw_record_t rec{};
     rec.num_records = R;
@@ -209,20 +238,60 @@ PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int nu
     rec.CaJSR = out.CaJSR.mutable_data(); rec.CaSS = out.CaSS.mutable_data(); rec.LCC = out.LCC.mutable_data();
     rec.LCC_inactivation = out.LCC_inactivation.mutable_data(); rec.ClCh = out.ClCh.mutable_data();
     rec.RyR = out.RyR.mutable_data();
-    const gw_globals_t g0 = init_globals.to_c();
-    Arr<double> CaSS = init_crus.CaSS, CaJSR = init_crus.CaJSR;
-    Arr<int32_t> LCC = init_crus.LCC, LCCi = init_crus.LCC_inactivation, RyR = init_crus.RyR, ClCh = init_crus.ClCh;
-    gw_cru_state_t c0{const_cast<double *>(CaSS.data()), const_cast<double *>(CaJSR.data()),
-                      const_cast<int32_t *>(LCC.data()), const_cast<int32_t *>(LCCi.data()),
-                      const_cast<int32_t *>(RyR.data()), const_cast<int32_t *>(ClCh.data()), nullptr};
     char err[1024] = {0};
     int rc;
     {
         py::gil_scoped_release release;
         std::lock_guard<std::mutex> lock(g_mutex);
         gw_ctx *ctx = context();
-        rc = gw_run(ctx, &params, nCRU, step_size, num_steps, istim.data(), record_every, &g0, &c0, s, prng,
-                    GW_PREC_FP64, &rec, err, sizeof(err));
+        rc = gw_run(ctx, &params, nCRU, step_size, num_steps, in.istim.data(), record_every, &in.g0, &in.c0, in.seed,
+                    in.prng, GW_PREC_FP64, &rec, err, sizeof(err));
+    }
+    if (rc != GW_OK) raise_status(rc, err);
+    return out;
+}
+
+// run() with its records in the reduced encoding of sim_dict_reduced_memory
+// (src/serialization.py:11-39), formed on the device: a dict with the same
+// keys, dtypes and shapes as sim_dict_reduced_memory(GWSolution(run(...))).
+py::dict run_reduced(const gw_params_t &params, int nCRU, double step_size, int num_steps, const py::object &Istim,
+                     int record_every, PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng,
+                     py::object seed)
+{
+    RunInput in = run_input(nCRU, step_size, num_steps, Istim, record_every, init_crus, init_globals, rng, seed);
+    const py::ssize_t R = in.records, N = nCRU;
+    py::dict out;
+    gw_record_reduced_t rec{};
+    rec.num_records = R;
+    const char *scalars[] = {"t", "V", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN", "CaHTRPN", "m", "h", "j", "xKs"};
+    float **fields[] = {&rec.t, &rec.V, &rec.Nai, &rec.Ki, &rec.Cai, &rec.CaNSR, &rec.CaLTRPN, &rec.CaHTRPN,
+                        &rec.m, &rec.h, &rec.j, &rec.xKs};
+    for (int k = 0; k < 12; ++k) {
+        auto a = zeros<float>({R});
+        *fields[k] = a.mutable_data();
+        out[scalars[k]] = a;
+    }
+    auto XKr = zeros<float>({R, 4}), XKv14 = zeros<float>({R, 9}), XKv43 = zeros<float>({R, 9});
+    auto CaSS = zeros<float>({R, N, 4});
+    auto CaJSR = zeros<float>({R, N});
+    auto LCC = zeros<int8_t>({R, N, 4});
+    auto LCCi = zeros<bool>({R, N, 4}), ClCh = zeros<bool>({R, N, 4});
+    auto RyR = zeros<int16_t>({R, N, 4});
+    rec.XKr = XKr.mutable_data(); rec.XKv14 = XKv14.mutable_data(); rec.XKv43 = XKv43.mutable_data();
+    rec.CaSS = CaSS.mutable_data(); rec.CaJSR = CaJSR.mutable_data(); rec.LCC = LCC.mutable_data();
+    rec.LCC_inactivation = reinterpret_cast<uint8_t *>(LCCi.mutable_data());
+    rec.ClCh = reinterpret_cast<uint8_t *>(ClCh.mutable_data());
+    rec.RyR = RyR.mutable_data();
+    out["XKr"] = XKr; out["XKv14"] = XKv14; out["XKv43"] = XKv43; out["CaSS"] = CaSS; out["CaJSR"] = CaJSR;
+    out["LCC"] = LCC; out["LCC_inactivation"] = LCCi; out["RyR"] = RyR; out["ClCh"] = ClCh;
+    char err[1024] = {0};
+    int rc;
+    {
+        py::gil_scoped_release release;
+        std::lock_guard<std::mutex> lock(g_mutex);
+        gw_ctx *ctx = context();
+        rc = gw_run_reduced(ctx, &params, nCRU, step_size, num_steps, in.istim.data(), record_every, &in.g0, &in.c0,
+                            in.seed, in.prng, GW_PREC_FP64, &rec, err, sizeof(err));
     }
     if (rc != GW_OK) raise_status(rc, err);
     return out;
@@ -313,6 +382,11 @@ PYBIND11_MODULE(GreensteinWinslow, m)
     m.def("run", &run, "Simulate the model", "parameters"_a, "nCRU"_a, "step_size"_a, "num_steps"_a, "Istim"_a,
           "record_every"_a, "init_crus"_a, "init_globals"_a = PyGlobalState(), "PRNG"_a = "mt19937_64",
           "seed"_a = py::none());
+
+    m.def("run_reduced", &run_reduced,
+          "run() with its records in the reduced encoding of sim_dict_reduced_memory, formed on the device",
+          "parameters"_a, "nCRU"_a, "step_size"_a, "num_steps"_a, "Istim"_a, "record_every"_a, "init_crus"_a,
+          "init_globals"_a = PyGlobalState(), "PRNG"_a = "mt19937_64", "seed"_a = py::none());
 
     m.def("rest_state", [](int nCRU, uint64_t seed, const std::string &rng) {
         PyCRUState s(nCRU);

@@ -166,6 +166,22 @@ class GWModel:
             raise
         return GWSolution(sol, self.parameters)
 
+    def simulate_reduced(self, step_size: float, num_steps: int, record_every: int = 1, PRNG: str = "xoshiro256++",
+                         seed: int | None = None) -> dict:
+        """simulate() with its records returned in the reduced encoding of
+        sim_dict_reduced_memory (src/serialization.py:11-39), formed on the
+        device: the same dictionary as sim_dict_reduced_memory(simulate(...))
+        for the same seed, without the full snapshots leaving the GPU."""
+        assert_positive(step_size, "step_size")
+        assert_positive(num_steps, "num_steps")
+        assert_positive(record_every, "record_every")
+        if PRNG not in IMPLEMENTED_PRNGS:
+            raise ValueError(f"{PRNG} is an invalid argument for PRNG.\nAvailable options are: "
+                             + ", ".join(self.PRNG_options()))
+        return gw_cxx.run_reduced(self.parameters.cxx_struct, self.parameters.NCaRU_sim, step_size, num_steps,
+                                  self.__stim, record_every, init_crus=self.__cru_state,
+                                  init_globals=self.__global_state, PRNG=PRNG, seed=seed)
+
     def stimulus_fn(self, t: float) -> float:
         return self.__stim(t)
 

@@ -150,6 +150,41 @@ typedef struct gw_record_t {
     int32_t *RyR;   /* [R][n][4][6] */
 } gw_record_t;
 
+/*
+ * The reduced encoding of the reference's sim_dict_reduced_memory
+ * (src/serialization.py:11-39), formed on the device so that only it crosses
+ * PCIe (40 bytes per CRU and record against 184): FP32 concentrations
+ * (round to nearest), the LCC state as int8, y and ClCh as 0/1 bytes (numpy
+ * bool), and per subunit the int16 RyR code of RyR_state_to_int16
+ * (src/utils.py:8-22): R1 + 6 R2 + 36 R3 + 216 R4 + 1296 R5, where a lumped
+ * pair (3,4) or (5,6) with a negative member counts as its sum in the first
+ * member. Any pointer may be NULL to skip that field.
+ */
+typedef struct gw_cru_state_reduced_t {
+    float   *CaSS;              /* [n][4] */
+    float   *CaJSR;             /* [n]    */
+    int8_t  *LCC;               /* [n][4] */
+    uint8_t *LCC_inactivation;  /* [n][4] */
+    uint8_t *ClCh;              /* [n][4] */
+    int16_t *RyR;               /* [n][4] */
+} gw_cru_state_reduced_t;
+
+/* Reduced snapshot sink: the globals in FP32 and, as in sim_dict_reduced_memory,
+ * only the first 4 of the 5 IKr states and the first 9 of the 10 Kv states
+ * (the last one of each chain is 1 minus the others). */
+typedef struct gw_record_reduced_t {
+    int64_t num_records;
+    float *t, *V, *m, *h, *j, *Nai, *Ki, *Cai, *CaNSR, *CaLTRPN, *CaHTRPN, *xKs;
+    float *XKr;     /* [R][4] */
+    float *XKv14;   /* [R][9] */
+    float *XKv43;   /* [R][9] */
+    float *CaJSR;   /* [R][n] */
+    float *CaSS;    /* [R][n][4] */
+    int8_t *LCC;    /* [R][n][4] */
+    uint8_t *LCC_inactivation, *ClCh;  /* [R][n][4] */
+    int16_t *RyR;   /* [R][n][4] */
+} gw_record_reduced_t;
+
 /* ---- stimulus / protocol of one cell ------------------------------------ */
 #define GW_PROTO_TABLE   0  /* Istim[k] read from a per-step table (reference run(), src/pyGW.hpp:114) */
 #define GW_PROTO_PACE    1  /* Istim = amplitude while mod(t - start, period) < duration, else 0 */
@@ -229,6 +264,16 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt,
            const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed,
            int32_t prng_id, int32_t precision, gw_record_t *out,
            char *err, size_t errlen);
+/*
+ * gw_run with its records in the reduced encoding (gw_record_reduced_t),
+ * replacing run() followed by sim_dict_reduced_memory (src/serialization.py:11-39)
+ * without the full snapshots ever leaving the device.
+ */
+int gw_run_reduced(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt,
+                   int64_t num_steps, const double *istim, int32_t record_every,
+                   const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed,
+                   int32_t prng_id, int32_t precision, gw_record_reduced_t *out,
+                   char *err, size_t errlen);
 
 /*
  * Batched session API: many independent cells (uneven CRU counts, per-cell
@@ -246,6 +291,9 @@ int gw_batch_set_cru_state(gw_batch *b, int32_t cell, const gw_cru_state_t *c,
                            char *err, size_t errlen);
 int gw_batch_get_cru_state(gw_batch *b, int32_t cell, gw_cru_state_t *c,
                            char *err, size_t errlen);
+/* One cell's CRU state in the reduced encoding of gw_cru_state_reduced_t. */
+int gw_batch_get_cru_state_reduced(gw_batch *b, int32_t cell, gw_cru_state_reduced_t *c,
+                                   char *err, size_t errlen);
 int gw_batch_set_globals(gw_batch *b, int32_t cell, const gw_globals_t *g,
                          char *err, size_t errlen);
 int gw_batch_get_globals(gw_batch *b, gw_globals_t *g_out /* [ncells] */,

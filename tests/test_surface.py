@@ -175,6 +175,37 @@ def test_ryr_int16_encoding(pkg):
     assert code.tolist() == [5, 6 + 72 + 216 + 1296, 6 + 36 * 3, 1 + 1296 * 4]
 
 
+def test_sim_dict_reduced_memory(pkg):
+    """The reduced record dictionary of src/serialization.py:11-39: FP32 globals
+    and concentrations, the first 4/9/9 states of the K chains, int8 LCC, bool
+    y/ClCh, int16 RyR codes; every other key passes through."""
+    R, N = 3, 2
+    rng = np.random.default_rng(0)
+    full = {k: rng.random(R) for k in ("t", "V", "Nai", "Ki", "Cai", "CaNSR", "CaLTRPN", "CaHTRPN", "m", "h", "j",
+                                        "xKs")}
+    full.update({"XKr": rng.random((R, 5)), "XKv14": rng.random((R, 10)), "XKv43": rng.random((R, 10)),
+                 "CaSS": rng.random((R, N, 4)), "CaJSR": rng.random((R, N)),
+                 "LCC": np.full((R, N, 4), 12, np.int32), "LCC_inactivation": np.ones((R, N, 4), np.int32),
+                 "ClCh": np.zeros((R, N, 4), np.int32),
+                 "RyR": np.tile(np.array([0, 1, 4, -1, 0, 0], np.int32), (R, N, 4, 1))})
+
+    class Sim:
+        def to_dict(self):
+            return {"params": {}, "vars": {k: v.copy() for k, v in full.items()}}
+
+    d = pkg.sim_dict_reduced_memory(Sim())
+    assert set(d) == set(full)
+    for k in ("t", "V", "Cai", "xKs", "CaSS", "CaJSR"):
+        assert d[k].dtype == np.float32 and d[k].shape == full[k].shape
+        assert np.array_equal(d[k], full[k].astype(np.float32))
+    assert d["XKr"].shape == (R, 4) and d["XKv14"].shape == (R, 9) and d["XKv43"].shape == (R, 9)
+    assert np.array_equal(d["XKv43"], full["XKv43"][:, :9].astype(np.float32))
+    assert d["LCC"].dtype == np.int8 and np.all(d["LCC"] == 12)
+    assert d["LCC_inactivation"].dtype == np.bool_ and d["LCC_inactivation"].all()
+    assert d["ClCh"].dtype == np.bool_ and not d["ClCh"].any()
+    assert d["RyR"].dtype == np.int16 and d["RyR"].shape == (R, N, 4) and np.all(d["RyR"] == 6 + 36 * 3)
+
+
 def test_numpy_validators_keep_reference_semantics(pkg):
     # raise only when every element violates the bound (src/utils.py:167,173)
     pkg.assert_gt_numpy(np.array([-1.0, 1.0]), 0, "x", strict=True)
