@@ -404,15 +404,19 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
         }
         QuadOut o{};
         uint64_t iters = 0;
+        int changed = 0;
         const R dt = static_cast<R>(a.dt);
         for (int64_t s = 0; s < nsteps; ++s) {
             o = ssa_step_quad<PRNG, DIGEST>(ca, jl, w, q, sp, kk, T, dt, dg, static_cast<uint32_t>((step0 + s) & 0xffff), j);
             iters += o.iters;
+            changed |= o.changed;
             if (o.err) break;
         }
         a.CaSS[slot * 4 + j] = ca;
-        JLCC[slot * 4 + j] = jl;
-        a.sub[slot * 4 + j] = w;
+        if (changed) {   // the channel word and JLCC change only at events and re-splits
+            JLCC[slot * 4 + j] = jl;
+            a.sub[slot * 4 + j] = w;
+        }
         if (j == 0) {
             a.CaJSR[slot] = q.jsr;
             a.rng[slot * 4] = q.r0;
@@ -638,10 +642,12 @@ __device__ __forceinline__ void lane_store(const KArgs &a, int64_t slot, const L
     double2 *qc = reinterpret_cast<double2 *>(a.CaSS + slot * 4);
     qc[0] = make_double2(S.ca[0], S.ca[1]);
     qc[1] = make_double2(S.ca[2], S.ca[3]);
-    *reinterpret_cast<float4 *>(a.JLCCf + slot * 4) = make_float4(S.jl[0], S.jl[1], S.jl[2], S.jl[3]);
-    ulonglong2 *qw = reinterpret_cast<ulonglong2 *>(a.sub + slot * 4);
-    qw[0] = make_ulonglong2(S.w[0], S.w[1]);
-    qw[1] = make_ulonglong2(S.w[2], S.w[3]);
+    if (S.changed) {   // the channel words and JLCC change only at events and re-splits
+        *reinterpret_cast<float4 *>(a.JLCCf + slot * 4) = make_float4(S.jl[0], S.jl[1], S.jl[2], S.jl[3]);
+        ulonglong2 *qw = reinterpret_cast<ulonglong2 *>(a.sub + slot * 4);
+        qw[0] = make_ulonglong2(S.w[0], S.w[1]);
+        qw[1] = make_ulonglong2(S.w[2], S.w[3]);
+    }
     ulonglong2 *qr = reinterpret_cast<ulonglong2 *>(a.rng + slot * 4);
     qr[0] = make_ulonglong2(S.q.r0, S.q.r1);
     qr[1] = make_ulonglong2(S.q.r2, S.q.r3);

@@ -75,6 +75,8 @@ struct SsaParamsT {
     R KBSR, KBSL, KCSQN, BSR_const, BSL_const, CSQN_const, VSS_VJSR;
     R k12, k21, k23, k32, k34, k43, k45, k54, k56, k65, k25, k52;
     R kfClCh, kbClCh, Cao_scaled;
+    // rate sums and products of the FP32-local subunit total (formed in FP64)
+    R k23_25, k45_43, k56_54, k45k34, k32k43, k65k52, k65k54;
 };
 using SsaParams = SsaParamsT<double>;
 using SsaParamsF = SsaParamsT<float>;
@@ -87,6 +89,8 @@ GW_HD void ssa_params_init(SsaParams &s, const gw_params_t &p, const CellConsts 
     s.k12 = p.k12; s.k21 = p.k21; s.k23 = p.k23; s.k32 = p.k32; s.k34 = p.k34; s.k43 = p.k43;
     s.k45 = p.k45; s.k54 = p.k54; s.k56 = p.k56; s.k65 = p.k65; s.k25 = p.k25; s.k52 = p.k52;
     s.kfClCh = p.kfClCh; s.kbClCh = p.kbClCh; s.Cao_scaled = c.Cao_scaled;
+    s.k23_25 = p.k23 + p.k25; s.k45_43 = p.k45 + p.k43; s.k56_54 = p.k56 + p.k54;
+    s.k45k34 = p.k45 * p.k34; s.k32k43 = p.k32 * p.k43; s.k65k52 = p.k65 * p.k52; s.k65k54 = p.k65 * p.k54;
 }
 
 GW_HD void ssa_params_f32(SsaParamsF &f, const SsaParams &s)
@@ -132,17 +136,37 @@ __device__ __forceinline__ void ryr_rates(const SsaParamsT<R> &P, uint64_t w, R 
     f(11, hi56 ? R(0) : static_cast<R>(n6) * P.k65);
 }
 
-// Subunit total in the reference accumulation order (SSA.tpp:48-56):
-// 0 + LCC sum, + inactivation, + RyR[0..11], + ClCh.
+// Subunit total. FP64: in the reference accumulation order (SSA.tpp:48-56),
+// 0 + LCC sum, + inactivation, + RyR[0..11], + ClCh, which bit-exact replay
+// needs. FP32-local: the same sum with the 12 RyR rates of ryr_rates gathered
+// by count (each lumped pair's two rates in one closed form), about half the
+// instructions; the event choice still walks the individual rates (ryr_pick).
 template <class R, class KT>
 __device__ __forceinline__ R subunit_total(const SsaParamsT<R> &P, const KT &K, const LccTableT<R> &T, uint64_t w, R ca)
 {
     const int L = sub_lcc(w);
     R s = T.P[L] + T.Q[L] * ca;
     s += sub_y(w) ? K.inact1 : K.inact0;
-    ryr_rates(P, w, ca, [&](int, R r) { s += r; });
-    s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
-    return s;
+    if constexpr (sizeof(R) == 8) {
+        ryr_rates(P, w, ca, [&](int, R r) { s += r; });
+        s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
+        return s;
+    } else {
+        s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
+        const R C2 = ca * ca;
+        const R n1 = static_cast<R>(ryr_get(w, 0)), n2 = static_cast<R>(ryr_get(w, 1));
+        const R n3 = static_cast<R>(ryr_get(w, 2)), n4 = static_cast<R>(ryr_get(w, 3));
+        const R n5 = static_cast<R>(ryr_get(w, 4)), n6 = static_cast<R>(ryr_get(w, 5));
+        // 1->2, 2->3, 2->5, 2->1
+        R r = C2 * (n1 * P.k12 + n2 * P.k23_25) + n2 * P.k21;
+        // 3<->4 with 4->5 and 3->2: lumped, (n3 + n4)(k45 k34 C2 + k32 k43) / (k34 C2 + k43)
+        if (ca > R(3.685e-2)) r += (n3 + n4) * ldiv(P.k45k34 * C2 + P.k32k43, P.k34 * C2 + P.k43);
+        else r += n3 * (P.k34 * C2 + P.k32) + n4 * P.k45_43;
+        // 5<->6 with 5->2 and 5->4: lumped, (n5 + n6) k65 (k52 + k54 C2) / (k56 C2 + k65)
+        if (ca > R(1.15e-4)) r += (n5 + n6) * ldiv(P.k65k52 + P.k65k54 * C2, P.k56 * C2 + P.k65);
+        else r += n5 * (P.k56_54 * C2 + P.k52) + n6 * P.k65;
+        return s + r;
+    }
 }
 
 // Rest-state sampler of one CRU: initialise_LCC / _LCC_i / _RyR / _ClCh
@@ -378,6 +402,7 @@ struct QuadOut {
     int clch;        // open ClCh after the step
     uint32_t iters;  // SSA loop iterations
     int err;         // nonzero if the iteration cap was hit
+    int changed;     // nonzero if this lane's subunit word or JLCC changed (an event or a re-split)
 };
 
 // Uniform draws: FP64 uses the reference's 64-bit map; FP32 the top 24 bits.
@@ -403,6 +428,7 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
     QuadOut out;
     out.iters = 0;
     out.err = 0;
+    out.changed = 0;
     const double Cai = kcai(K), CaNSR = kcansr(K);
     // copy_from_CRUState (SSA.tpp:10-33)
     int open = ryr_get(w, 2) + ryr_get(w, 3);
@@ -448,7 +474,10 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
                         if (qdraw<PRNG, R>(q) < pr) X++;
                     const int na = (n > 0) ? X : 0;
                     const int nb = (n > 0) ? n - X : 0;
-                    if (j == jj) w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
+                    if (j == jj) {
+                        w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
+                        out.changed = 1;
+                    }
                     if (DIGEST) digest_event(dig, step_lo, jj, bit == 1 ? 4 : 5, na);
                 }
             }
@@ -492,6 +521,7 @@ __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const 
         const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG, R>(q) : R(0);
         int detail = 0;
         if (j == js) {
+            out.changed = 1;
             const R caR = static_cast<R>(ca);
             if (cls == 0) {
                 // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
@@ -580,6 +610,7 @@ struct LaneState {
     float t, Jtr;
     uint32_t iters;
     int err;
+    int changed;   // the subunit words or JLCC changed this step (an event or a re-split)
 };
 
 // copy_from_CRUState (SSA.tpp:10-33): the step's starting fluxes.
@@ -596,6 +627,7 @@ __device__ __forceinline__ void lane_begin(LaneState &S, const SsaParamsF &P)
     S.Jtr = R(0);
     S.iters = 0;
     S.err = 0;
+    S.changed = 0;
 }
 
 // One iteration of SSA_single_CRU's loop (SSA.tpp:290-309); true when the
@@ -683,6 +715,7 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
             for (int i = 0; i < n; ++i)
                 if (qdraw<PRNG, R>(S.q) < pr) X++;
             S.w[j] = ryr_set(ryr_set(S.w[j], a_, (n > 0) ? X : 0), a_ + 1, (n > 0) ? n - X : 0);
+            S.changed = 1;
         }
         S.ca[j] = tmp;
     }
@@ -698,6 +731,7 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
         return true;
     }
     S.t += jump;
+    S.changed = 1;
     // apply the chosen event to subunit js after the update (SSA.tpp:262-283)
     uint64_t wj = sel4(S.w, js);
     const double caj_d = sel4(S.ca, js);
