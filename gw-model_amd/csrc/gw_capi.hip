@@ -606,6 +606,127 @@ __global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64
     ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, 1, T, acc);
 }
 
+// The SSA of block b with a quad of lanes per CRU and CPT CRUs per quad: wave
+// w of the block owns 16 * CPT consecutive CRUs of the block's cell; a quad
+// whose CRU has finished the step takes the next CRU not yet taken (the
+// finishing quads in quad order, by a ballot of their first lanes). Lane j
+// adds output j of each CRU it finishes to the exact block sums, so the
+// results do not depend on which quad ran which CRU. No replay digests.
+template <int PRNG, class R, int BLOCK, int CPT>
+__device__ __forceinline__ void ssa_block_refill(const KArgs &a, int b, int64_t step0, LccTableT<R> &T, BlockAcc &acc)
+{
+    constexpr int kQuads = kWave / 4;
+    constexpr int kMap = shape_of_crus(BLOCK / 4 * CPT);
+    static_assert(crus_per_block(kMap) == BLOCK / 4 * CPT, "CRUs per block must be 16 << map");
+    using LS = LocalState<R>;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int j = tid & 3;
+    const int qd = lane >> 2;
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const int ks = static_cast<int>(step0 & 1);
+    const DevCell &C = a.cells[cell];
+    const CellDyn &D = a.dyn[cell];
+    const auto &sp = LS::params(C);
+    const auto &kk = LS::consts(D, ks);
+    load_lcc_table(kk, T);
+    acc_zero(acc);
+    __syncthreads();
+    const int64_t w0 = static_cast<int64_t>(b - C.blk0[kMap]) * (BLOCK / 4 * CPT) + wave * (kQuads * CPT);
+    const int64_t w1 = min(w0 + kQuads * CPT, static_cast<int64_t>(C.n_cru));
+    int64_t i = w0 + qd;
+    int64_t next = w0 + kQuads;   // first CRU of the range not yet taken (wave-uniform)
+    bool have = (i < w1) && (D.status == 0);
+    R *const JLCC = LS::jlcc(a);
+    const R dt = static_cast<R>(a.dt);
+    const uint32_t step_lo = static_cast<uint32_t>(step0 & 0xffff);
+    QuadState<R> S;
+    auto load = [&](int64_t slot) {
+        S.ca = a.CaSS[slot * 4 + j];
+        S.jl = JLCC[slot * 4 + j];
+        S.w = a.sub[slot * 4 + j];
+        S.q.jsr = a.CaJSR[slot];
+        S.q.r0 = a.rng[slot * 4];
+        S.q.r1 = a.rng[slot * 4 + 1];
+        S.q.r2 = a.rng[slot * 4 + 2];
+        S.q.r3 = a.rng[slot * 4 + 3];
+        S.dig = Digest{0, 0};
+        quad_begin(S, sp);
+    };
+    if (have) load(C.slot0 + i);
+    while (__ballot(have)) {
+        bool fin = false;
+        if (have) {
+            fin = quad_iterate<PRNG, false>(S, sp, kk, T, dt, step_lo, j);
+            if (fin) {
+                const int64_t slot = C.slot0 + i;
+                a.CaSS[slot * 4 + j] = S.ca;
+                if (S.changed) {   // the channel word and JLCC change only at events and re-splits
+                    JLCC[slot * 4 + j] = S.jl;
+                    a.sub[slot * 4 + j] = S.w;
+                }
+                if (j == 0) {
+                    a.CaJSR[slot] = S.q.jsr;
+                    a.rng[slot * 4] = S.q.r0;
+                    a.rng[slot * 4 + 1] = S.q.r1;
+                    a.rng[slot * 4 + 2] = S.q.r2;
+                    a.rng[slot * 4 + 3] = S.q.r3;
+                }
+                const QuadOut o = quad_outputs(S);
+                // lane j adds output j of its CRU (update_CRUstate_from_temp, GW.tpp:47-62)
+                const double v = j == 0 ? o.Jxfer : j == 1 ? o.Jtr : j == 2 ? o.JLCC : o.Jrel;
+                Fx f;
+                fx_zero(f);
+                const int fl = fx_add(f, v);
+                acc_add_fx(acc, j, f);
+                const int e = (qgeti<0>(fl) | qgeti<1>(fl) | qgeti<2>(fl)) | (o.err ? 4 : 0);
+                const int64_t iv = j == 0 ? static_cast<int64_t>(o.iters) : j == 1 ? o.open : j == 2 ? o.clch : e;
+                atomicAdd(&acc.ints[j], static_cast<unsigned long long>(iv));
+            }
+        }
+        const uint64_t m = __ballot(fin && j == 0);
+        if (fin) {
+            i = next + static_cast<int64_t>(__popcll(m & ((1ULL << (4 * qd)) - 1)));
+            have = i < w1;
+            if (have) load(C.slot0 + i);
+        }
+        next += __popcll(m);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        Partial p;
+        p.jxfer = acc_fx(acc, 0);
+        p.jtr = acc_fx(acc, 1);
+        p.jlcc = acc_fx(acc, 2);
+        p.jrel = acc_fx(acc, 3);
+        p.iters = acc.ints[0];
+        p.open = static_cast<int64_t>(acc.ints[1]);
+        p.clch = static_cast<int64_t>(acc.ints[2]);
+        p.err = acc.ints[3] ? 1 : 0;
+        p._pad = 0;
+        a.partials[ks * a.nblocks + b] = p;
+    }
+}
+
+// The refill loop keeps a CRU's addresses and the next CRU's index live beside
+// the quad state, beyond the 128 VGPRs of 4 waves per SIMD; 3 waves per SIMD
+// keep it free of spills.
+template <int PRNG, class R, int BLOCK, int CPT>
+__global__ __launch_bounds__(BLOCK, 3) void k_step_refill(KArgs a, int64_t ssa_step, int32_t nclose,
+                                                           const int64_t *base)
+{
+    __shared__ LccTableT<R> T;
+    __shared__ CloseShared csh;
+    __shared__ BlockAcc acc;
+    const int bid = blockIdx.x;
+    if (bid < nclose) {
+        close_cell<kWave>(a, bid, csh);
+        return;
+    }
+    ssa_block_refill<PRNG, R, BLOCK, CPT>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc);
+}
+
 // Sum over the 64 lanes of a wave by a fixed xor butterfly (the same value,
 // bit for bit, in every lane).
 __device__ __forceinline__ double wave_sum(double v)
@@ -1063,10 +1184,11 @@ constexpr LaunchCfg kConfigs[kNumConfigs] = {
     {1, 64, 1},  {1, 128, 1}, {1, 256, 1},                               // one lane per CRU
     {1, 64, 2},  {1, 128, 2}, {1, 256, 2},                               // two CRUs per lane, refilled
     {1, 64, 4},  {1, 128, 4}, {1, 256, 4},                               // four
-    {1, 64, 8},  {1, 128, 8}, {1, 256, 8}};                              // eight (up to 2048 CRUs per block)
+    {1, 64, 8},  {1, 128, 8}, {1, 256, 8},                               // eight (up to 2048 CRUs per block)
+    {4, 64, 2},  {4, 64, 4},  {4, 128, 4}, {4, 256, 4}};                 // quads refilled from 2 or 4 CRUs each
 constexpr int config_shape(int c)
 {
-    return kConfigs[c].lanes == 4 ? shape_of_block(kConfigs[c].threads) : shape_of_crus(kConfigs[c].threads * kConfigs[c].cpl);
+    return shape_of_crus(kConfigs[c].threads / kConfigs[c].lanes * kConfigs[c].cpl);
 }
 static_assert(config_shape(kConfigDefault) == kShapeDefault, "the default configuration uses the setup map");
 static_assert(config_shape(kNumConfigs - 1) < kNumShapes, "block -> cell maps cover every configuration");
@@ -1088,6 +1210,7 @@ bool shape_allowed(const gw_batch *B, int c)
 {
     const bool fast = B->prng == GW_PRNG_XOSHIRO256SS && !B->frozen;
     if (kConfigs[c].lanes == 1) return fast && B->precision == GW_PREC_FP32_LOCAL;
+    if (kConfigs[c].cpl > 1) return fast && B->precision == GW_PREC_FP64 && !B->digest;   // refilled quads
     return c == kConfigDefault || (fast && (B->precision == GW_PREC_FP32_LOCAL || !B->digest));
 }
 
@@ -1189,6 +1312,16 @@ void launch_k_step(gw_batch *B, int64_t ssa_step, int32_t nclose, hipEvent_t e0,
             case 13: dispatch(k_step_lane<PRNG, 64, 8>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
             case 14: dispatch(k_step_lane<PRNG, 128, 8>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
             default: dispatch(k_step_lane<PRNG, 256, 8>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
+            }
+            return;
+        }
+        if (kConfigs[B->cfg].cpl > 1) {
+            const unsigned grid = step_grid(B, ssa_step, nclose);
+            switch (B->cfg) {   // FP64 only (shape_allowed)
+            case 16: dispatch(k_step_refill<PRNG, double, 64, 2>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 17: dispatch(k_step_refill<PRNG, double, 64, 4>, grid, 64, B, e0, e1, ssa_step, nclose, base); break;
+            case 18: dispatch(k_step_refill<PRNG, double, 128, 4>, grid, 128, B, e0, e1, ssa_step, nclose, base); break;
+            default: dispatch(k_step_refill<PRNG, double, 256, 4>, grid, 256, B, e0, e1, ssa_step, nclose, base); break;
             }
             return;
         }
@@ -1974,14 +2107,14 @@ int gw_batch_set_launch(gw_batch *B, int32_t lanes_per_cru, int32_t block, int32
         if (kConfigs[k].lanes == lanes && kConfigs[k].threads == threads && kConfigs[k].cpl == cpl) c = k;
     if (c < 0) {
         set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads x %d CRUs per lane is not available "
-                "(4 lanes: 64, 128, 256 or 512 threads, 1 CRU per lane; 1 lane: 64, 128 or 256 threads, 1, 2, 4 or 8 "
-                "CRUs per lane)", lanes_per_cru, block, crus_per_lane);
+                "(4 lanes: 64, 128, 256 or 512 threads with 1 CRU per quad, 64 with 2, 64, 128 or 256 with 4; "
+                "1 lane: 64, 128 or 256 threads, 1, 2, 4 or 8 CRUs per lane)", lanes_per_cru, block, crus_per_lane);
         return GW_E_INVALID_ARG;
     }
     if (!shape_allowed(B, c)) {
         set_err(err, errlen, "launch shape of %d lanes per CRU x %d threads is not available for this batch (other "
                 "shapes than 4 x 256 need xoshiro256** without replay digests or frozen globals; 1 lane per CRU needs "
-                "FP32-local)", lanes, threads);
+                "FP32-local; refilled quads need FP64)", lanes, threads);
         return GW_E_INVALID_ARG;
     }
     set_shape(B, c);

@@ -109,6 +109,30 @@ __device__ __forceinline__ double lneglog(double u) { return -gw::log(u); }
 __device__ __forceinline__ float lneglog(float u) { return -(__builtin_amdgcn_logf(u) * 0.693147182f); }
 __device__ __forceinline__ float sq(float x) { return x * x; }
 
+// Rapid-buffering factors of update_CaSS (SSA.tpp:90) and update_CaJSR
+// (SSA.hpp:69). FP64: the reference's expressions. FP32-local: the same
+// fractions over one denominator, one reciprocal each instead of three and two.
+template <class R>
+__device__ __forceinline__ R beta_ss(const SsaParamsT<R> &P, R ca)
+{
+    if constexpr (sizeof(R) == 8) {
+        return ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + ca)) + ldiv(P.BSL_const, sq(P.KBSL + ca)));
+    } else {
+        const R x2 = sq(P.KBSR + ca), y2 = sq(P.KBSL + ca), p = x2 * y2;
+        return ldiv(p, p + P.BSR_const * y2 + P.BSL_const * x2);
+    }
+}
+template <class R>
+__device__ __forceinline__ R beta_jsr(const SsaParamsT<R> &P, R jsr)
+{
+    if constexpr (sizeof(R) == 8) {
+        return ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + jsr)));
+    } else {
+        const R x2 = sq(P.KCSQN + jsr);
+        return ldiv(x2, x2 + P.CSQN_const);
+    }
+}
+
 // The 12 RyR propensities of one subunit (update_RyR_rates, GW_utils.tpp:222-255),
 // visited in storage order; `f(k, r)` receives each rate.
 template <class R, class F>
@@ -421,158 +445,208 @@ __device__ __forceinline__ R qdraw(CruQ &q)
 // FP32-local variant); KT the step constants of that type. CaSS (`ca`) and
 // CaJSR accumulate in FP64 in both; with R = double every expression below is
 // the reference's, in its order.
+template <class R>
+struct QuadState {
+    double ca;      // this lane's subunit CaSS (FP64 accumulator in both forms)
+    R jl;           // this lane's subunit JLCC
+    uint64_t w;     // this lane's subunit word
+    CruQ q;         // CaJSR and the CRU's engine (the same in the four lanes)
+    Digest dig;     // replay digest (the same in the four lanes)
+    int open;
+    R Jrel, t, Jtr, Jxfer;
+    uint32_t iters;
+    int err, changed;
+};
+
+// copy_from_CRUState (SSA.tpp:10-33): the step's starting fluxes.
+template <class R>
+__device__ __forceinline__ void quad_begin(QuadState<R> &S, const SsaParamsT<R> &P)
+{
+    S.iters = 0;
+    S.err = 0;
+    S.changed = 0;
+    S.open = ryr_get(S.w, 2) + ryr_get(S.w, 3);
+    S.Jrel = P.rRyR * static_cast<R>(S.open) * static_cast<R>(S.q.jsr - S.ca);
+    S.t = R(0);
+    S.Jtr = R(0);
+    S.Jxfer = R(0);
+}
+
+// One iteration of SSA_single_CRU's loop (SSA.tpp:290-309) for a quad; true
+// when the step is finished (the same in the four lanes).
+template <int PRNG, bool DIGEST, class R, class KT>
+__device__ __forceinline__ bool quad_iterate(QuadState<R> &S, const SsaParamsT<R> &P, const KT &K,
+                                             const LccTableT<R> &T, R dt, uint32_t step_lo, int j)
+{
+    const double Cai = kcai(K), CaNSR = kcansr(K);
+    S.iters++;
+    // update_fluxes + update_rates (SSA.tpp:37-56)
+    const double oldA = S.ca;
+    const R old = static_cast<R>(oldA);
+    const double o_next = qnext(oldA);
+    const double o_prev = qprev(oldA);
+    S.Jtr = P.rtr * static_cast<R>(CaNSR - S.q.jsr);
+    const R s_me = subunit_total(P, K, T, S.w, old);
+    const R s0 = qget<0>(s_me), s1 = qget<1>(s_me), s2 = qget<2>(s_me), s3 = qget<3>(s_me);
+    const R total = s0 + s1 + s2 + s3;
+    const R jump = ldiv(lneglog(qdraw<PRNG, R>(S.q)), total);
+    const bool last = !(S.t + jump < dt);
+    const R h = last ? dt - S.t : jump;
+    const uint64_t w_pre = S.w;
+    // update_CaSS (SSA.tpp:87-99) for this lane's subunit
+    const R Jiss = P.riss * static_cast<R>(o_next + o_prev - 2 * oldA);
+    S.Jxfer = P.rxfer * static_cast<R>(oldA - Cai);
+    const R betaSS = beta_ss(P, old);
+    const R dCaSS = h * (S.jl + S.Jrel - S.Jxfer + Jiss) * betaSS;
+    const double tmp = oldA + static_cast<double>(dCaSS);
+    const int split = split_mask<R>(oldA, tmp);
+    const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
+    if (any_split) {
+        // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
+        // lane draws the same Bernoulli uniforms and the owner applies them
+        for (int jj = 0; jj < 4; ++jj) {
+            const int sj = qseli(split, jj);
+            for (int bit = 1; bit <= 2; bit <<= 1) {
+                if (!(sj & bit)) continue;
+                const int a_ = (bit == 1) ? 4 : 2;
+                const uint64_t wq = qselu(S.w, jj);
+                const R oq = qsel(old, jj);
+                const R pr = (bit == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(oq)) : ldiv(P.k43, P.k43 + P.k34 * sq(oq));
+                const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
+                int X = 0;
+                for (int i = 0; i < n; ++i)
+                    if (qdraw<PRNG, R>(S.q) < pr) X++;
+                const int na = (n > 0) ? X : 0;
+                const int nb = (n > 0) ? n - X : 0;
+                if (j == jj) {
+                    S.w = ryr_set(ryr_set(S.w, a_, na), a_ + 1, nb);
+                    S.changed = 1;
+                }
+                if (DIGEST) digest_event(S.dig, step_lo, jj, bit == 1 ? 4 : 5, na);
+            }
+        }
+    }
+    S.ca = tmp;
+    // update_CaJSR (SSA.hpp:67-71), replicated
+    {
+        const R Jsum = qsum4(S.Jrel);
+        const R betaJSR = beta_jsr(P, static_cast<R>(S.q.jsr));
+        S.q.jsr += static_cast<double>(h * betaJSR * (S.Jtr - P.VSS_VJSR * Jsum));
+    }
+    if (last) return true;
+    if (S.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
+        S.err = 1;
+        return true;
+    }
+    S.t += jump;
+    // sample_weights over the subunits (common.hpp:49-59)
+    int js;
+    {
+        const R u = qdraw<PRNG, R>(S.q) * total;
+        R cum = s0;
+        js = 0;
+        if (cum < u) { js = 1; cum += s1;
+            if (cum < u) { js = 2; cum += s2;
+                if (cum < u) { js = 3; } } }
+    }
+    // sample_new_state (SSA.tpp:262-283): every lane classifies its own
+    // subunit; the chosen lane's class is broadcast to the quad
+    const R u3 = qdraw<PRNG, R>(S.q);
+    const int L = sub_lcc(S.w);
+    const int y = sub_y(S.w);
+    const int cl = sub_cl(S.w);
+    const R sum_LCC = T.P[L] + T.Q[L] * old;
+    const R inact = y ? K.inact1 : K.inact0;
+    const R clr = cl ? P.kbClCh : P.kfClCh * old;
+    const R sum_RyR = s_me - (sum_LCC + inact + clr);
+    const R u = u3 * s_me;
+    const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
+    const int cls_q = qseli(cls, js);
+    const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG, R>(S.q) : R(0);
+    int detail = 0;
+    if (j == js) {
+        S.changed = 1;
+        const R caR = static_cast<R>(S.ca);
+        if (cls == 0) {
+            // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
+            const R r0 = T.R0[L];
+            const R r1 = (L == 1) ? T.Q[L] * old : T.R1[L];
+            const R uu = u4 * sum_LCC;
+            R cum = r0;
+            int tr = 0;
+            if (cum < uu) { tr = 1; cum += r1;
+                if (cum < uu) { tr = 2; } }
+            const int Ln = lcc_next(L, tr);
+            if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
+                S.jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR) : R(0);
+            else if (L == 6 || L == 12)
+                S.jl = R(0);
+            S.w = sub_set_lcc(S.w, Ln);
+            detail = Ln;
+        } else if (cls == 1) {
+            const int yn = 1 - y;
+            S.w = sub_flip_y(S.w);
+            if (yn == 0) S.jl = R(0);
+            else if (L == 6 || L == 12) S.jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR);
+            detail = yn;
+        } else if (cls == 2) {
+            // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
+            int tr = ryr_pick(P, w_pre, old, u4 * sum_RyR);
+            if (sizeof(R) == 4 && tr >= 0 && !ryr_allowed(S.w, tr, S.ca)) tr = -1;
+            if (tr >= 0) {
+                if (!ryr_in_range(S.w, tr)) S.err = 1;
+                S.w = ryr_apply(S.w, tr);
+            }
+            if (tr == 1 || tr == 10) S.open += 1;
+            else if (tr == 4 || tr == 7) S.open -= 1;
+            S.Jrel = P.rRyR * static_cast<R>(S.open) * static_cast<R>(S.q.jsr - S.ca);
+            detail = tr;
+        } else {
+            S.w = sub_flip_cl(S.w);
+            detail = 1 - cl;
+        }
+    }
+    if (DIGEST) digest_event(S.dig, step_lo, js, cls_q, qseli(detail, js));
+    return false;
+}
+
+// outputs of update_CRUstate_from_temp (GW.tpp:47-62) for the reductions;
+// FP32 values enter the FP64 sums exactly
+template <class R>
+__device__ __forceinline__ QuadOut quad_outputs(const QuadState<R> &S)
+{
+    QuadOut out;
+    out.Jxfer = qsum4(static_cast<double>(S.Jxfer));
+    out.Jtr = static_cast<double>(S.Jtr);
+    out.JLCC = qsum4(static_cast<double>(S.jl));
+    out.Jrel = qsum4(static_cast<double>(S.Jrel));
+    out.open = qsum4i(ryr_get(S.w, 2) + ryr_get(S.w, 3));
+    out.clch = qsum4i(sub_cl(S.w));
+    out.iters = S.iters;
+    out.err = S.err;
+    out.changed = S.changed;
+    return out;
+}
+
 template <int PRNG, bool DIGEST, class R, class KT>
 __device__ QuadOut ssa_step_quad(double &ca, R &jl, uint64_t &w, CruQ &q, const SsaParamsT<R> &P,
                                  const KT &K, const LccTableT<R> &T, R dt, Digest &dig, uint32_t step_lo, int j)
 {
-    QuadOut out;
-    out.iters = 0;
-    out.err = 0;
-    out.changed = 0;
-    const double Cai = kcai(K), CaNSR = kcansr(K);
-    // copy_from_CRUState (SSA.tpp:10-33)
-    int open = ryr_get(w, 2) + ryr_get(w, 3);
-    R Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
-    R t = R(0), Jtr = R(0), Jxfer = R(0);
-    while (true) {
-        out.iters++;
-        // update_fluxes + update_rates (SSA.tpp:37-56)
-        const double oldA = ca;
-        const R old = static_cast<R>(oldA);
-        const double o_next = qnext(oldA);
-        const double o_prev = qprev(oldA);
-        Jtr = P.rtr * static_cast<R>(CaNSR - q.jsr);
-        const R s_me = subunit_total(P, K, T, w, old);
-        const R s0 = qget<0>(s_me), s1 = qget<1>(s_me), s2 = qget<2>(s_me), s3 = qget<3>(s_me);
-        const R total = s0 + s1 + s2 + s3;
-        const R jump = ldiv(lneglog(qdraw<PRNG, R>(q)), total);
-        const bool last = !(t + jump < dt);
-        const R h = last ? dt - t : jump;
-        const uint64_t w_pre = w;
-        // update_CaSS (SSA.tpp:87-99) for this lane's subunit
-        const R Jiss = P.riss * static_cast<R>(o_next + o_prev - 2 * oldA);
-        Jxfer = P.rxfer * static_cast<R>(oldA - Cai);
-        const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old)) + ldiv(P.BSL_const, sq(P.KBSL + old)));
-        const R dCaSS = h * (jl + Jrel - Jxfer + Jiss) * betaSS;
-        const double tmp = oldA + static_cast<double>(dCaSS);
-        const int split = split_mask<R>(oldA, tmp);
-        const int any_split = qgeti<0>(split) | qgeti<1>(split) | qgeti<2>(split) | qgeti<3>(split);
-        if (any_split) {
-            // sample_RyR56 / sample_RyR34 (SSA.tpp:59-84), in subunit order; every
-            // lane draws the same Bernoulli uniforms and the owner applies them
-            for (int jj = 0; jj < 4; ++jj) {
-                const int sj = qseli(split, jj);
-                for (int bit = 1; bit <= 2; bit <<= 1) {
-                    if (!(sj & bit)) continue;
-                    const int a_ = (bit == 1) ? 4 : 2;
-                    const uint64_t wq = qselu(w, jj);
-                    const R oq = qsel(old, jj);
-                    const R pr = (bit == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(oq)) : ldiv(P.k43, P.k43 + P.k34 * sq(oq));
-                    const int n = ryr_get(wq, a_) + ryr_get(wq, a_ + 1);
-                    int X = 0;
-                    for (int i = 0; i < n; ++i)
-                        if (qdraw<PRNG, R>(q) < pr) X++;
-                    const int na = (n > 0) ? X : 0;
-                    const int nb = (n > 0) ? n - X : 0;
-                    if (j == jj) {
-                        w = ryr_set(ryr_set(w, a_, na), a_ + 1, nb);
-                        out.changed = 1;
-                    }
-                    if (DIGEST) digest_event(dig, step_lo, jj, bit == 1 ? 4 : 5, na);
-                }
-            }
-        }
-        ca = tmp;
-        // update_CaJSR (SSA.hpp:67-71), replicated
-        {
-            const R Jsum = qsum4(Jrel);
-            const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(q.jsr))));
-            q.jsr += static_cast<double>(h * betaJSR * (Jtr - P.VSS_VJSR * Jsum));
-        }
-        if (last) break;
-        if (out.iters >= kMaxItersPerStep) {   // no physical state gets here; stop instead of spinning
-            out.err = 1;
-            break;
-        }
-        t += jump;
-        // sample_weights over the subunits (common.hpp:49-59)
-        int js;
-        {
-            const R u = qdraw<PRNG, R>(q) * total;
-            R cum = s0;
-            js = 0;
-            if (cum < u) { js = 1; cum += s1;
-                if (cum < u) { js = 2; cum += s2;
-                    if (cum < u) { js = 3; } } }
-        }
-        // sample_new_state (SSA.tpp:262-283): every lane classifies its own
-        // subunit; the chosen lane's class is broadcast to the quad
-        const R u3 = qdraw<PRNG, R>(q);
-        const int L = sub_lcc(w);
-        const int y = sub_y(w);
-        const int cl = sub_cl(w);
-        const R sum_LCC = T.P[L] + T.Q[L] * old;
-        const R inact = y ? K.inact1 : K.inact0;
-        const R clr = cl ? P.kbClCh : P.kfClCh * old;
-        const R sum_RyR = s_me - (sum_LCC + inact + clr);
-        const R u = u3 * s_me;
-        const int cls = (u < sum_LCC) ? 0 : (u < (sum_LCC + inact)) ? 1 : (u < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
-        const int cls_q = qseli(cls, js);
-        const R u4 = (cls_q == 0 || cls_q == 2) ? qdraw<PRNG, R>(q) : R(0);
-        int detail = 0;
-        if (j == js) {
-            out.changed = 1;
-            const R caR = static_cast<R>(ca);
-            if (cls == 0) {
-                // sample_LCC (SSA.tpp:103-197); the scan never compares after adding r2
-                const R r0 = T.R0[L];
-                const R r1 = (L == 1) ? T.Q[L] * old : T.R1[L];
-                const R uu = u4 * sum_LCC;
-                R cum = r0;
-                int tr = 0;
-                if (cum < uu) { tr = 1; cum += r1;
-                    if (cum < uu) { tr = 2; } }
-                const int Ln = lcc_next(L, tr);
-                if ((L == 5 && Ln == 6) || (L == 11 && Ln == 12))
-                    jl = (y == 1) ? K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR) : R(0);
-                else if (L == 6 || L == 12)
-                    jl = R(0);
-                w = sub_set_lcc(w, Ln);
-                detail = Ln;
-            } else if (cls == 1) {
-                const int yn = 1 - y;
-                w = sub_flip_y(w);
-                if (yn == 0) jl = R(0);
-                else if (L == 6 || L == 12) jl = K.JLCC_mult * (P.Cao_scaled - K.JLCC_exp * caR);
-                detail = yn;
-            } else if (cls == 2) {
-                // sample_RyR (SSA.tpp:200-259) over the propensities before this iteration's update
-                int tr = ryr_pick(P, w_pre, old, u4 * sum_RyR);
-                if (sizeof(R) == 4 && tr >= 0 && !ryr_allowed(w, tr, ca)) tr = -1;
-                if (tr >= 0) {
-                    if (!ryr_in_range(w, tr)) out.err = 1;
-                    w = ryr_apply(w, tr);
-                }
-                if (tr == 1 || tr == 10) open += 1;
-                else if (tr == 4 || tr == 7) open -= 1;
-                Jrel = P.rRyR * static_cast<R>(open) * static_cast<R>(q.jsr - ca);
-                detail = tr;
-            } else {
-                w = sub_flip_cl(w);
-                detail = 1 - cl;
-            }
-        }
-        if (DIGEST) digest_event(dig, step_lo, js, cls_q, qseli(detail, js));
+    QuadState<R> S;
+    S.ca = ca;
+    S.jl = jl;
+    S.w = w;
+    S.q = q;
+    S.dig = dig;
+    quad_begin(S, P);
+    while (!quad_iterate<PRNG, DIGEST>(S, P, K, T, dt, step_lo, j)) {
     }
-    // outputs of update_CRUstate_from_temp (GW.tpp:47-62) for the reductions;
-    // FP32 values enter the FP64 sums exactly
-    out.Jxfer = qsum4(static_cast<double>(Jxfer));
-    out.Jtr = static_cast<double>(Jtr);
-    out.JLCC = qsum4(static_cast<double>(jl));
-    out.Jrel = qsum4(static_cast<double>(Jrel));
-    out.open = qsum4i(ryr_get(w, 2) + ryr_get(w, 3));
-    out.clch = qsum4i(sub_cl(w));
-    return out;
+    ca = S.ca;
+    jl = S.jl;
+    w = S.w;
+    q = S.q;
+    dig = S.dig;
+    return quad_outputs(S);
 }
 
 // ---------------------------------------------------------------------------
@@ -700,7 +774,7 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
     for (int j = 0; j < 4; ++j) {
         const R Jiss = P.riss * static_cast<R>(oldA[(j + 1) & 3] + oldA[(j + 3) & 3] - 2 * oldA[j]);
         S.Jxfer[j] = P.rxfer * static_cast<R>(oldA[j] - Cai);
-        const R betaSS = ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + old[j])) + ldiv(P.BSL_const, sq(P.KBSL + old[j])));
+        const R betaSS = beta_ss(P, old[j]);
         const R dCaSS = h * (S.jl[j] + S.Jrel[j] - S.Jxfer[j] + Jiss) * betaSS;
         const double tmp = oldA[j] + static_cast<double>(dCaSS);
         int split = split_mask<R>(oldA[j], tmp);
@@ -722,7 +796,7 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
     // update_CaJSR (SSA.hpp:67-71)
     {
         const R Jsum = S.Jrel[0] + S.Jrel[1] + S.Jrel[2] + S.Jrel[3];
-        const R betaJSR = ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + static_cast<R>(S.q.jsr))));
+        const R betaJSR = beta_jsr(P, static_cast<R>(S.q.jsr));
         S.q.jsr += static_cast<double>(h * betaJSR * (S.Jtr - P.VSS_VJSR * Jsum));
     }
     if (last) return true;

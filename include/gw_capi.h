@@ -337,15 +337,16 @@ int gw_batch_get_iterations(gw_batch *b, uint64_t *iters);
  * Launch shape of the SSA kernel. lanes_per_cru: 4 (a quad of lanes per CRU,
  * one subunit per lane) or 1 (one lane per CRU, FP32-local batches only);
  * 0 = 4. block: threads per block, 64, 128, 256 or 512 with 4 lanes, 64, 128
- * or 256 with 1 lane; 0 = 256 or 64. crus_per_lane (1 lane only): 1, 2, 4 or
- * 8; each wave then owns 64 x crus_per_lane consecutive CRUs and a lane whose
- * CRU has finished its step takes the next one, so waves wait less on their
- * slowest CRU; 0 = 1. FP64 results do not depend on the shape (exact flux
+ * or 256 with 1 lane; 0 = 256 or 64. crus_per_lane: 1, 2, 4 or 8 with 1 lane
+ * (64, 128 or 256 threads); 1 with 4 lanes, or 2 (64 threads) and 4 (64, 128
+ * or 256 threads); each wave then owns 64 x crus_per_lane consecutive CRUs (16 x
+ * with quads) and a lane or quad whose CRU has finished its step takes the next
+ * one, so waves wait less on their slowest CRU; 0 = 1. FP64 results do not depend on the shape (exact flux
  * sums). Shapes other than 4 x 256 exist for xoshiro256** batches without
  * replay digests or frozen globals. No reference counterpart (the reference
  * has one OpenMP loop over CRUs, src/includes/GW.tpp:223-233).
  */
-#define GW_LAUNCH_CONFIGS 16
+#define GW_LAUNCH_CONFIGS 20
 /* The shape of launch configuration `index` (0 .. GW_LAUNCH_CONFIGS-1), the order of autotune's timings. */
 int gw_launch_config(int32_t index, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane);
 int gw_batch_set_launch(gw_batch *b, int32_t lanes_per_cru, int32_t block, int32_t crus_per_lane,

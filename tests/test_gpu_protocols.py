@@ -156,16 +156,20 @@ def _snapshot(b, ncells):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_launch_shapes_give_identical_results(gwcapi, gpu_ctx, precision):
-    """Every SSA block size gives the same bits (exact flux sums, per-CRU streams)."""
+    """Every quad launch shape, refilled or not, gives the same bits (exact flux
+    sums, per-CRU streams)."""
     say(f"launch shapes, precision {precision}")
     pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
     specs = ((37, 1), (300, 2), (1000, 3))
     ref = None
-    for blk in (64, 128, 256, 512):
+    shapes = ((64, 1), (128, 1), (256, 1), (512, 1))
+    if precision == 0:   # refilled quads are built for the FP64 form
+        shapes += ((64, 2), (64, 4), (128, 4), (256, 4))
+    for blk, cpl in shapes:
         b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, s, None, pace) for n, s in specs], precision=precision)
         b.set_mode(frozen=False, digest=False)
-        b.set_launch(4, blk)
-        assert b.launch() == (4, blk, 1)
+        b.set_launch(4, blk, cpl)
+        assert b.launch() == (4, blk, cpl)
         b.step(1500)
         snap = _snapshot(b, len(specs))
         b.close()
@@ -173,7 +177,7 @@ def test_launch_shapes_give_identical_results(gwcapi, gpu_ctx, precision):
             ref = snap
             continue
         for x, y in zip(snap[0], ref[0]):
-            assert np.array_equal(x, y), blk
+            assert np.array_equal(x, y), (blk, cpl)
         for x, y in zip(snap[1], ref[1]):
             assert_states_equal(x, y)
 
@@ -187,8 +191,8 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
     a.step(300)
     shape, ms = a.autotune(40)
     say(f"  chose {shape}: {ms}")
-    assert shape in ((4, 64, 1), (4, 128, 1), (4, 256, 1), (4, 512, 1)) and a.launch() == shape
-    assert all(ms[k] is not None and ms[k] > 0 for k in ("4x64", "4x128", "4x256", "4x512"))
+    assert shape[0] == 4 and a.launch() == shape   # quads, refilled or not: FP64
+    assert all(ms[k] is not None and ms[k] > 0 for k in ("4x64", "4x128", "4x256", "4x512", "4x64x2", "4x64x4"))
     lane = [k for k in ms if k.startswith("1x")]
     assert len(lane) == 12 and all(ms[k] is None for k in lane)  # one lane per CRU is FP32-local only
     a.step(300)
