@@ -17,14 +17,21 @@ algorithm).
   roofline     for the step kernel k_step: algorithmic flops per launch
                (F_ITER per SSA iteration x iterations counted in-kernel) and
                algorithmic state bytes per launch (bytes per CRU-step x CRUs),
-               each divided by the kernel's average duration from the start /
-               end events every dispatch of the timed region records; against
-               the vector peak of the form's precision and HBM. `traffic` is
+               each divided by the kernel's average duration (the timed region
+               between its two HIP events over its SSA launches, which run
+               back to back); against the vector peak of the form's precision
+               and HBM. `traffic` is
                the measured HBM bytes per launch from
                profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and WRITE_SIZE,
                calibrated on a copy of known size) when present.
   cpu_baseline the CPU restatement built with the reference's release flags,
-               OpenMP over CRUs on all host cores, on a bounded sample
+               OpenMP over CRUs on all host cores, on a bounded sample; a
+               one-thread sample of a smaller cell over the same window
+               beside it (per-core throughput), and the host CPU model
+  launch       the SSA launch shape: by default the autotuner's choice on
+               this batch (1x64x4 FP32-local, 4x64 FP64), fixed so that a
+               profile of the same command sees the same kernels;
+               --autotune STEPS runs the tuner instead
 """
 from __future__ import annotations
 
@@ -170,11 +177,28 @@ def measured_traffic(total_crus, form):
         return None
 
 
-def cpu_baseline(crus, steps, warmup_steps=0, seed=42):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(crus, steps, warmup_steps=0, seed=42, threads=None):
+    """CRU*ms/s of the CPU restatement (OpenMP over CRUs) on one cell of `crus`
+    CRUs over `steps` steps from t = 0; `threads` caps the OpenMP team (None:
+    every host core)."""
     sys.path.insert(0, ROOT)
     from oracle.oracle import OracleModel, lib
 
     L = lib("omp")
+    gomp = C.CDLL("libgomp.so.1")
+    all_threads = L.gwo_model_threads()
+    gomp.omp_set_num_threads(int(threads or all_threads))
     threads = L.gwo_model_threads()
     m = OracleModel(crus, seed=seed, kind="omp")
     m.sample_rest_state()
@@ -188,6 +212,7 @@ def cpu_baseline(crus, steps, warmup_steps=0, seed=42):
     t0 = time.perf_counter()
     rc = L.gwo_model_run(m.h, 1e-3, steps, ist[warmup_steps:].ctypes.data_as(C.c_void_p), 0, None)
     dt = time.perf_counter() - t0
+    gomp.omp_set_num_threads(int(all_threads))
     if rc != 0:
         raise RuntimeError(f"oracle run failed: {rc}")
     return crus * steps * 1e-3 / dt, threads, dt
@@ -217,7 +242,7 @@ def measure(capi, precision, args, total_crus):
     f32 = precision == capi.PREC_FP32_LOCAL
     cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
     fixed = args.launch_fp32 if f32 else args.launch_fp64
-    launch = tuple(int(x) for x in (fixed + "x1").split("x")[:3]) if fixed else None
+    launch = tuple(int(x) for x in (fixed + "x1").split("x")[:3]) if fixed and args.autotune <= 0 else None
     ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, cell_list, args.steps, args.warmup,
                                                           precision=precision, autotune=args.autotune, launch=launch)
     sim_ms = args.steps * 1e-3
@@ -263,15 +288,19 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--crus", type=int, default=12500)
     ap.add_argument("--cpu-crus", type=int, default=12500)
-    ap.add_argument("--cpu-steps", type=int, default=20000)
+    ap.add_argument("--cpu-steps", type=int, default=40000)
+    ap.add_argument("--cpu-crus-1t", type=int, default=1500, help="CRUs of the one-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="seeds64",
                     help="seeds64 (default): the seed ensemble; batch256 / large1024: the uneven and the large batch")
-    ap.add_argument("--autotune", type=int, default=200, metavar="STEPS",
-                    help="autotuner trial steps per SSA block size before the timed region (0: default shape)")
-    ap.add_argument("--launch-fp32", default=None, metavar="LxB[xC]",
-                    help="fixed FP32-local launch shape (lanes per CRU x threads x CRUs per lane) instead of the autotuner")
-    ap.add_argument("--launch-fp64", default=None, metavar="LxB", help="fixed FP64 launch shape instead of the autotuner")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="run the autotuner with STEPS trial steps per launch shape, on a copy of the batch advanced "
+                         "to the middle of the timed window, and time its choice instead of the fixed shapes")
+    ap.add_argument("--launch-fp32", default="1x64x4", metavar="LxB[xC]",
+                    help="FP32-local launch shape (lanes per CRU x threads x CRUs per lane); the default is the "
+                         "autotuner's choice on the seeds64 batch")
+    ap.add_argument("--launch-fp64", default="4x64", metavar="LxB",
+                    help="FP64 launch shape; the default is the autotuner's choice on the seeds64 batch")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -325,9 +354,13 @@ def main(argv=None):
     value = head["value"]
     if not args.no_cpu and rank == 0:
         v, threads, secs = cpu_baseline(args.cpu_crus, args.cpu_steps)
+        v1, _, secs1 = cpu_baseline(args.cpu_crus_1t, args.cpu_steps, threads=1)
         line["cpu_baseline"] = {"value": v, "unit": "CRU\u00b7ms/s", "cores": threads, "kind": "port",
                                 "sample": f"1 cell x {args.cpu_crus} CRUs x {args.cpu_steps} steps "
                                           f"(t = 0..{args.cpu_steps * 1e-3:g} ms of beat 1), {secs:.1f} s",
+                                "cpu_model": cpu_model(),
+                                "one_thread": {"value": v1, "sample": f"1 cell x {args.cpu_crus_1t} CRUs x "
+                                                                     f"{args.cpu_steps} steps, {secs1:.1f} s"},
                                 "ratio_gpu_over_cpu": value / v,
                                 "ratio_fp64_gpu_over_cpu": res["fp64"]["value"] / v if "fp64" in res else None}
     if rank == 0:
