@@ -28,7 +28,7 @@ def test_engines_raw_outputs(oracle_lib, golden, prng):
 
 
 def test_known_answer_xoshiro256ss_seed42(oracle_lib):
-    # Xoshiro256StarStar(42), first three outputs (also stated in the survey of the reference).
+    # Xoshiro256StarStar(42), first three outputs (SplitMix64 seeding of xoshiro.hpp:684-703).
     s = np.zeros(4, np.uint64)
     oracle_lib.gwo_rng_seed(0, s.ctypes.data, 42)
     got = [oracle_lib.gwo_rng_next(0, s.ctypes.data) for _ in range(3)]
