@@ -100,6 +100,12 @@ _LIBS: dict[str, C.CDLL] = {}
 
 
 def lib(kind: str = "strict") -> C.CDLL:
+    """The oracle library of one build: "strict" (IEEE, the checker), "par"
+    (the same with OpenMP over CRUs) or "omp" (the reference's release flags,
+    the timed CPU baseline). Loading "omp" sets flush-to-zero and
+    denormals-are-zero for the whole process (its -Ofast start-up code, as
+    with the reference's own -Ofast module), so checks that need IEEE
+    subnormals must not share a process with it."""
     name = {"strict": "liboracle.so", "par": "liboracle_par.so", "omp": "liboracle_omp.so"}[kind]
     if kind in _LIBS:
         return _LIBS[kind]
