@@ -310,8 +310,9 @@ def main(argv=None):
         copy_calibration(args.copy_calibration)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(cells are independent and would shard by cell, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (cells are independent: "
+              "several GPUs would each take a range of whole cells, with no exchange during the run)",
+              file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
