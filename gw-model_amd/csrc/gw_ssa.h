@@ -440,6 +440,18 @@ __device__ __forceinline__ R qdraw(CruQ &q)
     }
 }
 
+// Two uniforms from one 64-bit draw, bits 40..63 and 16..39 (the one-lane
+// FP32-local layout, built for xoshiro256** only, whose scrambled output has
+// no weak bit): an event iteration then advances the engine twice, not four
+// times.
+template <int PRNG>
+__device__ __forceinline__ void draw2(CruQ &q, float &hi, float &mid)
+{
+    const uint64_t x = rng_next<PRNG>(q.r0, q.r1, q.r2, q.r3);
+    hi = static_cast<float>(static_cast<uint32_t>(x >> 40)) * 0x1p-24f;
+    mid = static_cast<float>(static_cast<uint32_t>(x >> 16) & 0xffffffu) * 0x1p-24f;
+}
+
 // One global step of SSA_single_CRU (SSA.tpp:286-310); lane j holds subunit j.
 // R is the local arithmetic type (double: the reference path; float: the
 // FP32-local variant); KT the step constants of that type. CaSS (`ca`) and
@@ -711,6 +723,8 @@ __device__ __forceinline__ void lane_begin(LaneState &S, const SsaParamsF &P)
 // rather than after them: the draws are independent, so the joint law of the
 // choice and the re-splits is the same, and only the choice (not the
 // pre-update channel words and totals) has to be kept across the update.
+// The four uniforms of an event iteration (waiting time, subunit, class,
+// transition) come two per engine draw (draw2).
 template <int PRNG>
 __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, const StepConstsF &K,
                                              const LccTableF &T, float dt)
@@ -735,12 +749,14 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
 #pragma unroll
         for (int j = 0; j < 4; ++j) s[j] = subunit_total(P, K, T, S.w[j], old[j]);
         const R total = s[0] + s[1] + s[2] + s[3];
-        jump = ldiv(lneglog(qdraw<PRNG, R>(S.q)), total);
+        R u_jump, u_sub;
+        draw2<PRNG>(S.q, u_jump, u_sub);
+        jump = ldiv(lneglog(u_jump), total);
         last = !(S.t + jump < dt);
         h = last ? dt - S.t : jump;
         if (!last) {
             // sample_weights over the subunits (common.hpp:49-59)
-            const R u = qdraw<PRNG, R>(S.q) * total;
+            const R u = u_sub * total;
             R cum = s[0];
             if (cum < u) { js = 1; cum += s[1];
                 if (cum < u) { js = 2; cum += s[2];
@@ -754,18 +770,20 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
             const R inact = sub_y(wj) ? K.inact1 : K.inact0;
             const R clr = sub_cl(wj) ? P.kbClCh : P.kfClCh * oldj;
             const R sum_RyR = sj - (sum_LCC + inact + clr);
-            const R u3 = qdraw<PRNG, R>(S.q) * sj;
+            R u_cls, u_tr;
+            draw2<PRNG>(S.q, u_cls, u_tr);
+            const R u3 = u_cls * sj;
             cls = (u3 < sum_LCC) ? 0 : (u3 < (sum_LCC + inact)) ? 1 : (u3 < (sum_LCC + inact + sum_RyR)) ? 2 : 3;
             if (cls == 0) {
                 // sample_LCC (SSA.tpp:103-197)
                 const R r0 = T.R0[L];
                 const R r1 = (L == 1) ? T.Q[L] * oldj : T.R1[L];
-                const R uu = qdraw<PRNG, R>(S.q) * sum_LCC;
+                const R uu = u_tr * sum_LCC;
                 R c = r0;
                 if (c < uu) { tr = 1; c += r1;
                     if (c < uu) { tr = 2; } }
             } else if (cls == 2) {
-                tr = ryr_pick(P, wj, oldj, qdraw<PRNG, R>(S.q) * sum_RyR);
+                tr = ryr_pick(P, wj, oldj, u_tr * sum_RyR);
             }
         }
     }
