@@ -724,7 +724,8 @@ __device__ __forceinline__ void lane_begin(LaneState &S, const SsaParamsF &P)
 // choice and the re-splits is the same, and only the choice (not the
 // pre-update channel words and totals) has to be kept across the update.
 // The four uniforms of an event iteration (waiting time, subunit, class,
-// transition) come two per engine draw (draw2).
+// transition) and the Bernoulli trials of a re-split come two per engine draw
+// (draw2).
 template <int PRNG>
 __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, const StepConstsF &K,
                                              const LccTableF &T, float dt)
@@ -804,8 +805,12 @@ __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, 
             const R pr = (bit == 1) ? ldiv(P.k65, P.k65 + P.k56 * sq(old[j])) : ldiv(P.k43, P.k43 + P.k34 * sq(old[j]));
             const int n = ryr_get(S.w[j], a_) + ryr_get(S.w[j], a_ + 1);
             int X = 0;
-            for (int i = 0; i < n; ++i)
-                if (qdraw<PRNG, R>(S.q) < pr) X++;
+            for (int i = 0; i < n; i += 2) {   // two Bernoulli trials per engine draw
+                R b0, b1;
+                draw2<PRNG>(S.q, b0, b1);
+                X += (b0 < pr) ? 1 : 0;
+                X += (i + 1 < n && b1 < pr) ? 1 : 0;
+            }
             S.w[j] = ryr_set(ryr_set(S.w[j], a_, (n > 0) ? X : 0), a_ + 1, (n > 0) ? n - X : 0);
             S.changed = 1;
         }
