@@ -224,6 +224,23 @@ def launch_configs() -> list[tuple[int, int, int]]:
     return out
 
 
+def rest_state(n_cru: int, seed: int, prng: int = 0) -> dict:
+    """gw_rest_state: the seeded rest state of one cell's CRUs, sampled on the
+    host (initialise_LCC / _LCC_i / _RyR / _ClCh, GW_utils.tpp:5-64); a batch
+    samples the same state on the device."""
+    n = int(n_cru)
+    out = {"CaSS": np.zeros((n, 4)), "CaJSR": np.zeros(n), "LCC": np.zeros((n, 4), np.int32),
+           "LCC_inactivation": np.zeros((n, 4), np.int32), "RyR": np.zeros((n, 4, 6), np.int32),
+           "ClCh": np.zeros((n, 4), np.int32), "JLCC": np.zeros((n, 4))}
+    s = CruState(*[_ptr(out[k]) for k in ("CaSS", "CaJSR", "LCC", "LCC_inactivation", "RyR", "ClCh", "JLCC")])
+    L = lib()
+    L.gw_rest_state.argtypes = [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(CruState)]
+    rc = L.gw_rest_state(n, int(seed) & (2**64 - 1), int(prng), C.byref(s))
+    if rc != GW_OK:
+        raise GwError(rc, f"gw_rest_state: bad arguments (n_cru {n}, prng {prng})")
+    return out
+
+
 def launch_name(lanes: int, block: int, crus_per_lane: int = 1) -> str:
     """'4x256' (lanes per CRU x threads), '1x64x4' with CRUs per lane above 1."""
     return f"{lanes}x{block}" + (f"x{crus_per_lane}" if crus_per_lane > 1 else "")

@@ -82,6 +82,10 @@ def test_host_rest_state_matches_oracle(gwcapi, prng):
     ref = m.state()
     for k in out:
         assert np.array_equal(out[k], ref[k]), k
+    # the binding's wrapper returns the same arrays
+    via = gwcapi.rest_state(n, seed, prng)
+    for k in out:
+        assert np.array_equal(via[k], out[k]), k
     # the rest-state occupancies are those of the reference sampler's weights
     assert set(np.unique(out["LCC"])) <= {1, 2, 7}
     assert np.all(out["RyR"].sum(axis=2) == 5)
