@@ -155,6 +155,30 @@ def test_fp32_local_channel_counts_stay_valid_at_scale(gwcapi, gpu_ctx, launch):
     b.close()
 
 
+def test_fp32_local_whole_beat_at_scale(gwcapi, gpu_ctx):
+    """64 cells x 12,500 CRUs through a whole paced beat (500 ms, about 30 s on
+    one GPU): every channel count stays valid, every cell repolarises with an
+    APD90 in the range of the FP64 ensemble, and the cells return to rest."""
+    cells = [gwcapi.make_cell(12500, 42 + s, None, PACE) for s in range(64)]
+    b = gwcapi.Batch(gpu_ctx, cells, precision=gwcapi.PREC_FP32_LOCAL)
+    b.set_mode(frozen=False, digest=False)
+    b.set_launch(1, 64, 4)
+    for k in range(5):
+        say(f"fp32-local whole beat at scale, t = {k * 100}..{(k + 1) * 100} ms")
+        b.step(100_000)
+    for c in range(len(cells)):
+        R = b.get_cru_state(c)["RyR"]
+        assert np.all(R.sum(axis=2) == 5)
+        assert np.all(R[..., 0] >= 0) and np.all(R[..., 1] >= 0)
+        assert np.all(R[..., 2] + R[..., 3] >= 0) and np.all(R[..., 4] + R[..., 5] >= 0)
+    apd = np.array([o["apd90_last"] for o in b.obs()])
+    v = np.array([x.V for x in b.get_globals()])
+    say(f"  APD90 {apd.min():.1f}..{apd.max():.1f} ms, final V {v.min():.2f}..{v.max():.2f} mV")
+    assert np.all((apd > 250) & (apd < 450))
+    assert np.all(v < -85.0)
+    b.close()
+
+
 def test_fp32_local_clamp_family_errors(gwcapi, gpu_ctx):
     """Voltage-clamp steps from -80 mV to -40..+40 mV: peak ICaL and the
     integrated RyR release flux of the FP32-local form agree with FP64 within
