@@ -118,6 +118,7 @@ __device__ __forceinline__ R beta_ss(const SsaParamsT<R> &P, R ca)
     if constexpr (sizeof(R) == 8) {
         return ldiv(R(1), 1 + ldiv(P.BSR_const, sq(P.KBSR + ca)) + ldiv(P.BSL_const, sq(P.KBSL + ca)));
     } else {
+#pragma clang fp contract(fast)
         const R x2 = sq(P.KBSR + ca), y2 = sq(P.KBSL + ca), p = x2 * y2;
         return ldiv(p, p + P.BSR_const * y2 + P.BSL_const * x2);
     }
@@ -128,6 +129,7 @@ __device__ __forceinline__ R beta_jsr(const SsaParamsT<R> &P, R jsr)
     if constexpr (sizeof(R) == 8) {
         return ldiv(R(1), 1 + ldiv(P.CSQN_const, sq(P.KCSQN + jsr)));
     } else {
+#pragma clang fp contract(fast)
         const R x2 = sq(P.KCSQN + jsr);
         return ldiv(x2, x2 + P.CSQN_const);
     }
@@ -176,6 +178,7 @@ __device__ __forceinline__ R subunit_total(const SsaParamsT<R> &P, const KT &K, 
         s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
         return s;
     } else {
+#pragma clang fp contract(fast)
         s += sub_cl(w) ? P.kbClCh : P.kfClCh * ca;
         const R C2 = ca * ca;
         const R n1 = static_cast<R>(ryr_get(w, 0)), n2 = static_cast<R>(ryr_get(w, 1));
@@ -702,6 +705,7 @@ struct LaneState {
 // copy_from_CRUState (SSA.tpp:10-33): the step's starting fluxes.
 __device__ __forceinline__ void lane_begin(LaneState &S, const SsaParamsF &P)
 {
+#pragma clang fp contract(fast)   // FP32-local only: fused multiply-adds allowed
     using R = float;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -730,6 +734,7 @@ template <int PRNG>
 __device__ __forceinline__ bool lane_iterate(LaneState &S, const SsaParamsF &P, const StepConstsF &K,
                                              const LccTableF &T, float dt)
 {
+#pragma clang fp contract(fast)   // FP32-local only: fused multiply-adds allowed
     using R = float;
     const double Cai = K.Cai_d, CaNSR = K.CaNSR_d;
     S.iters++;

@@ -55,11 +55,12 @@ _FP64_ENSEMBLE = {}
 
 @pytest.mark.parametrize("launch", [(4, 256, 1), (1, 64, 1), (1, 64, 4)], ids=["quad", "lane", "lane-refill"])
 def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx, launch):
-    """64 seeds, one paced beat (500 ms, well past the longest APD90 of about
-    410 ms, so no cell's beat is cut off): APD90, peak Cai, peak ICaL and mean
-    RyR open fraction agree within 3 SE, for a quad of lanes per CRU, for one
-    lane per CRU and for lanes refilled from four CRUs each."""
-    seeds, n, steps = 64, 1000, 500_000
+    """64 seeds, one paced beat (600 ms: past the longest APD90 of the FP64
+    form, about 420 ms over 256 seeds, and of the FP32-local form's longer
+    tail, about 500 ms, so no cell's beat is cut off): APD90, peak Cai, peak
+    ICaL and mean RyR open fraction agree within 3 SE, for a quad of lanes per
+    CRU, for one lane per CRU and for lanes refilled from four CRUs each."""
+    seeds, n, steps = 64, 1000, 600_000
     cells = [gwcapi.make_cell(n, 500 + s, None, PACE) for s in range(seeds)]
     if "o64" not in _FP64_ENSEMBLE:
         say(f"fp64 ensemble {seeds} x {n} CRUs x {steps} steps")
