@@ -295,3 +295,32 @@ def test_on_device_apd_and_alternans_match_the_trace(gwcapi, gpu_ctx):
         assert abs(o["alt_sum"] / o["alt_count"] - float(np.mean(np.abs(np.diff(apds))))) < 0.02
         assert o["peak_Cai"] >= tr[:, c, 1].max() and o["peak_ICaL_neg"] <= tr[:, c, 2].min()
         assert abs(o["sum_open_RyR_frac"] / o["n_steps"] - tr[:, c, 3].mean()) < 2e-3
+
+
+@pytest.mark.parametrize("field, value, codes", [("Nai", -1.0, ("GW_E_NONPOSITIVE_CONC",)),
+                                                  ("V", float("nan"), ("GW_E_NAN", "GW_E_RANGE"))],
+                         ids=["nonpositive-conc", "nan"])
+def test_a_failing_cell_stops_alone(gwcapi, gpu_ctx, field, value, codes):
+    """A cell whose globals become invalid stops with its own status, and the
+    step call reports it, naming the cell, instead of ending the process (the
+    reference calls exit() on a non-positive Nernst argument, common.hpp:72-75).
+    The other cells of the batch run on bit-identical to a batch without it."""
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    cells = [gwcapi.make_cell(256, 70 + k, None, pace) for k in range(3)]
+    say(f"failing cell: {field} = {value}")
+    ref = gwcapi.Batch(gpu_ctx, cells)
+    ref.step(2000)
+    want = [g.as_array() for g in ref.get_globals()]
+    ref.close()
+    b = gwcapi.Batch(gpu_ctx, cells)
+    g = b.get_globals()[1]
+    setattr(g, field, value)
+    b.set_globals(1, g)
+    with pytest.raises(gwcapi.GwError) as e:
+        b.step(2000)
+    assert gwcapi.STATUS[e.value.code] in codes
+    assert "cell 1" in str(e.value)
+    got = [g.as_array() for g in b.get_globals()]
+    for c in (0, 2):
+        assert np.array_equal(got[c], want[c]), c
+    b.close()
