@@ -213,6 +213,38 @@ def test_uneven_batch_matches_independent_oracles(gwcapi, gpu_ctx):
         assert_states_equal(b.get_cru_state(c), m.state())
 
 
+def test_extreme_cell_sizes(gwcapi, gpu_ctx):
+    """CRU counts at both ends of the supported range, 1 and 2^20, advance in
+    one batch exactly as the oracle; 0 and 2^20 + 1 are rejected."""
+    pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
+    for bad in (0, (1 << 20) + 1):
+        with pytest.raises(gwcapi.GwError) as e:
+            gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(bad, 1, None, pace)])
+        assert gwcapi.STATUS[e.value.code] == "GW_E_INVALID_ARG"
+    specs = [(1, 21), (1 << 20, 22)]
+    steps = 20
+    say("extreme cell sizes 1 and 2^20")
+    b = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(n, seed, None, pace) for n, seed in specs])
+    b.set_mode(frozen=False, digest=True)
+    b.step(steps)
+    gg = b.get_globals()
+    ist = np.zeros(steps)
+    t = 0.0
+    for k in range(steps):
+        ist[k] = 100.0 if t < 1.0 else 0.0
+        t += 1e-3
+    for c, (n, seed) in enumerate(specs):
+        m = OracleModel(n, seed=seed, kind="par")   # the strict restatement with OpenMP over CRUs
+        m.sample_rest_state()
+        m.run(1e-3, steps, ist)
+        assert_globals_equal(gg[c], m.globals())
+        c_g, h_g = b.digest(c)
+        c_o, h_o = m.digest()
+        assert np.array_equal(c_g, c_o) and np.array_equal(h_g, h_o)
+        assert_states_equal(b.get_cru_state(c), m.state())
+    b.close()
+
+
 def test_full_size_cell_invariants_and_determinism(gwcapi, gpu_ctx):
     """12,500-CRU cell: channel-state invariants hold after an upstroke, and
     the same seed reproduces the same bits."""
