@@ -179,6 +179,28 @@ def test_gw_run_records_match_oracle(gwcapi, gpu_ctx):
         assert np.array_equal(out_g[k], v), k
 
 
+@pytest.mark.parametrize("steps,rec,rows", [(0, 10, 0), (3, 1000, 1), (10, 10, 1)], ids=["empty", "short", "exact"])
+def test_gw_run_degenerate_lengths(gwcapi, gpu_ctx, steps, rec, rows):
+    """gw_run with no steps, with fewer steps than record_every and with
+    exactly one record interval: ceil(num_steps / record_every) rows, each
+    equal to the oracle's."""
+    n = 20
+    say(f"gw_run steps={steps} record_every={rec}")
+    m = OracleModel(n, seed=5)
+    m.sample_rest_state()
+    crus = m.state()
+    istim = _pulse(steps)
+    g0 = gwcapi.default_globals()
+    out_g = gwcapi.run(gpu_ctx, gwcapi.Params.default(), n, 1e-3, steps, istim, rec, g0, crus, seed=5)
+    assert out_g["t"].shape[0] == rows
+    m2 = OracleModel(n, seed=5)
+    m2.set_state(globals_=to_oracle_globals(g0), **{k: crus[k] for k in STATE_KEYS if k != "JLCC"})
+    out_o = m2.run(1e-3, steps, istim, record_every=rec)
+    for k, v in out_o.items():
+        assert out_g[k].shape == v.shape, k
+        assert np.array_equal(out_g[k], v), k
+
+
 def test_uneven_batch_matches_independent_oracles(gwcapi, gpu_ctx):
     """Cells with different sizes, parameters, seeds and pacing advance in one
     launch exactly as each would alone."""
