@@ -21,10 +21,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # -ffp-contract=off: fused multiply-adds only where the source asks (bit-exact FP64 path);
-# packed FP32 instructions disabled.
+# packed FP32 instructions disabled; the instruction scheduler set to favour
+# instruction-level parallelism (it reorders independent instructions only, so
+# results are unchanged; the step kernels ran 1.0% (FP32-local) and 1.5% (FP64)
+# faster with it).
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-sched-strategy=max-ilp",
+    "-Wall", "-Wno-unused-function",
 ]
 
 
