@@ -78,3 +78,31 @@ def test_capi_run_reduced_matches_run(gwcapi, gpu_ctx, pkg):
     want = pkg.sim_dict_reduced_memory(Sim())
     for k in want:
         assert red[k].dtype == want[k].dtype and np.array_equal(red[k], want[k]), k
+
+
+def test_gw_run_fp32_local_records(gwcapi, gpu_ctx, pkg):
+    """gw_run and gw_run_reduced in the FP32-local form: the same seed gives
+    the same run, so the reduced records formed on the device equal the host
+    encoding of the full records; every recorded channel state is valid."""
+    n, steps, rec = 1000, 20_000, 1000
+    say("gw_run FP32-local records")
+    istim = np.where(np.arange(steps) * 1e-3 < 1.0, 100.0, 0.0)
+    args = (gpu_ctx, gwcapi.Params.default(), n, 1e-3, steps, istim, rec, gwcapi.default_globals(), None)
+    full = gwcapi.run(*args, seed=31, precision=gwcapi.PREC_FP32_LOCAL)
+    red = gwcapi.run_reduced(*args, seed=31, precision=gwcapi.PREC_FP32_LOCAL)
+    R = full["RyR"]
+    assert R.shape == (steps // rec, n, 4, 6)
+    assert np.all(R.sum(axis=3) == 5)
+    assert np.all((full["LCC"] >= 1) & (full["LCC"] <= 12))
+    assert np.all(np.isfinite(full["CaSS"])) and np.all(full["CaSS"] > 0)
+    assert full["V"].max() > 0.0   # the stimulus produced an upstroke
+
+    class _Sim:
+        def to_dict(self):
+            return {"vars": {k: np.copy(v) for k, v in full.items()}}
+
+    host = pkg.sim_dict_reduced_memory(_Sim())
+    assert set(red) == set(host)
+    for k in host:
+        assert red[k].dtype == host[k].dtype, k
+        assert np.array_equal(red[k], host[k]), k
