@@ -29,18 +29,28 @@
  * Pinning (how each part is checked beyond this restatement):
  *   xoshiro/splitmix/uniform map/sample_weights/sample_binomial -> golden vectors
  *       from the reference's own xoshiro.hpp and common.hpp compiled in oracle/_ref
- *   common.hpp currents and gates -> golden vectors from oracle/_ref (<= 4 ulp)
+ *   common.hpp currents and gates (INa, INaK, INaCa, IpCa, Ib, Nernst, m/h/j
+ *       rates) -> golden vectors from oracle/_ref (<= 8 ulp, because of D1)
  *   libm -> numpy/glibc over sweeps (<= 2 ulp) and exact identities
- *   LCC / RyR rate tables -> detailed balance (Kolmogorov cycle criterion) of
- *       the published channel models; CTMC stationary statistics of the SSA
- *   Q matrices -> zero row sums; published initial Kr/Kv vectors are near
- *       steady state at the published rest potential
- *   global derivatives -> published rest state is near equilibrium (|dV/dt|,
- *       |dNai/dt|, |dKi/dt|, |dCai/dt| small); closed-form special cases
- *   SSA waiting times -> mean 1/total rate; stationary occupancy vs scipy CTMC
- *   GW_utils.hpp currents (IKr, IKs, IKv14, IK1, IKp, Ito2, Jup, beta_cyto):
- *       pinned only through the rest-state equilibrium and closed-form limits
- *       (the header needs Eigen, which is absent, so no golden vectors exist).
+ *   GW_utils.hpp gating functions (LCC alpha/beta/yinf/tau, XKsinf, tauXKs)
+ *       -> special values (Boltzmann midpoints, e-fold points, limits and
+ *       one-term asymptotes), tests/test_oracle_pins.py
+ *   GW_utils.hpp currents (IKr, IK1, IKp, IKv43, IKv14, Jup) -> special values
+ *       and limits; reversal potentials; GHK zero flux (test_oracle_pins.py,
+ *       test_oracle_model.py)
+ *   LCC / RyR rate tables -> the four-sensor structure entry by entry;
+ *       detailed balance (Kolmogorov cycle criterion); the stationary LCC law
+ *       at rest against the published rest sampler; CTMC statistics of the SSA
+ *   Q matrices -> every rate at V = 0 and its voltage ratios; the Kv
+ *       four-sensor structure; zero row sums; the published initial Kr/Kv
+ *       vectors near the stationary law at the published rest potential
+ *   global derivatives -> membrane charge = cytosolic ion charge; pump and
+ *       exchanger stoichiometry; compartment Ca balance of the SR fluxes; the
+ *       master equation of each chain; the published rest state near
+ *       equilibrium; whole-cell Ca conservation during release
+ *   SSA waiting times -> mean 1/total rate; stationary occupancy vs CTMC
+ *   The pins catch planted mistakes: tests/test_oracle_mutations.py builds 20
+ *   mutated copies of this file and requires a pin to fail on each.
  */
 #include "gw_oracle.h"
 

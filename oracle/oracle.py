@@ -112,6 +112,13 @@ def lib(kind: str = "strict") -> C.CDLL:
     path = os.path.join(BUILD, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+    _LIBS[kind] = load(path)
+    return _LIBS[kind]
+
+
+def load(path: str) -> C.CDLL:
+    """Load an oracle build from `path` and declare its signatures (lib() for
+    the three builds of oracle/Makefile; tests also load mutated builds)."""
     L = C.CDLL(path)
     L.gwo_splitmix64_next.restype = u64
     L.gwo_splitmix64_next.argtypes = [C.POINTER(u64)]
@@ -170,7 +177,6 @@ def lib(kind: str = "strict") -> C.CDLL:
     L.gwo_model_iterations.restype = u64
     L.gwo_model_iterations.argtypes = [P]
     L.gwo_model_threads.restype = C.c_int
-    _LIBS[kind] = L
     return L
 
 
