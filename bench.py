@@ -1,33 +1,38 @@
 #!/usr/bin/env python3
 """Throughput of the Greenstein-Winslow coupled integrator on one MI355X.
 
-Workload: 64 independent seeds (64 cells) of a 12,500-CRU cell, 1 Hz pacing
-with a +100 pA/pF, 1 ms stimulus (the reference's sign: positive is
-depolarising), dt = 1e-3 ms, starting from the seeded rest state. One bench
-step is one Euler step of the whole batch (every cell: global ODE + SSA of all
-its CRUs). W warmup steps run untimed first, then exactly K steps are timed
-with HIP events on the library's stream. Both forms run on the same batch:
-the batched FP32-local form (CRU-local arithmetic in FP32, global ODE and flux
-sums in FP64) and the plain FP64 form (bit-exact with the reference's
-algorithm).
+Default workload: 1,024 cells x 12,500 CRUs (12.8 M CRUs), each cell with
+its own seed and with PCaL, GKr, GKs, GK1, GKv43 and kNaCa scaled
+log-uniform in [0.5, 2], paced at 2 Hz with a +100 pA/pF, 1 ms stimulus (the
+reference's sign: positive is depolarising), dt = 1e-3 ms, from the seeded rest
+state. One bench step is one Euler step of the whole batch (every cell: global
+ODE + SSA of all its CRUs). The batch is first advanced untimed to the start of
+the timed window (--window-start-ms, default 1 ms: the upstroke, so the window
+holds the upstroke and the Ca release, the busiest part of the beat), or by
+--warmup steps if that is further; then exactly K steps are timed with HIP
+events on the library's stream. The timed window is stated in `config`. Both
+forms run on the same batch: the batched FP32-local form (CRU-local arithmetic
+in FP32, global ODE and flux sums in FP64) and the plain FP64 form (bit-exact
+with the reference's algorithm).
 
   value        CRU*ms simulated per second over the whole batch, FP32-local
                form (the plain FP64 form is under "fp64")
-  cell_beats   cells * (simulated ms / 1000 ms per beat) per second
-  roofline     for the step kernel k_step: algorithmic flops per launch
-               (F_ITER per SSA iteration x iterations counted in-kernel) and
-               algorithmic state bytes per launch (bytes per CRU-step x CRUs),
-               each divided by the kernel's average duration (the timed region
-               between its two HIP events over its SSA launches, which run
-               back to back); against the vector peak of the form's precision
-               and HBM. `traffic` is
-               the measured HBM bytes per launch from
+  cell_beats   cells * (simulated ms / 500 ms per beat) per second
+  roofline     for the step kernel: algorithmic flops per launch (F_ITER per
+               SSA iteration x iterations counted in-kernel) and algorithmic
+               state bytes per launch (bytes per CRU-step x CRUs), each divided
+               by the kernel's average duration (the timed region between its
+               two HIP events over its SSA launches, which run back to back);
+               against the vector peak of the form's precision and HBM.
+               `traffic` is the measured HBM bytes per launch from
                profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and WRITE_SIZE,
                calibrated on a copy of known size) when present.
   cpu_baseline the CPU restatement built with the reference's release flags,
-               OpenMP over CRUs on all host cores, on a bounded sample; a
-               one-thread sample of a smaller cell over the same window
-               beside it (per-core throughput), and the host CPU model
+               OpenMP over CRUs on all host cores, on a bounded sample: the
+               batch's first cell (its parameters and seed) over the same
+               timed window; a one-thread sample of a smaller cell over the
+               same window beside it (per-core throughput), and the host CPU
+               model
   launch       the SSA launch shape: by default the autotuner's choice on
                this batch (1x64x4 FP32-local, 4x64 FP64), fixed so that a
                profile of the same command sees the same kernels;
@@ -104,6 +109,7 @@ def make_cells(capi, workload, cells, crus):
 
 
 def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=None):
+    """Time `steps` Euler steps of the batch after `warmup` untimed ones."""
     ctx = capi.Context(0)
     tune = None
     if launch is not None:   # a fixed launch shape instead of the autotuner's choice
@@ -188,24 +194,27 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(crus, steps, warmup_steps=0, seed=42, threads=None):
+def cpu_baseline(crus, steps, warmup_steps=0, seed=42, threads=None, params=None, period=1000.0):
     """CRU*ms/s of the CPU restatement (OpenMP over CRUs) on one cell of `crus`
-    CRUs over `steps` steps from t = 0; `threads` caps the OpenMP team (None:
-    every host core)."""
+    CRUs over `steps` steps after `warmup_steps` untimed ones from t = 0 (paced
+    with +100 pA/pF for 1 ms every `period` ms); `threads` caps the OpenMP team
+    (None: every host core); `params` is the cell's parameter set (a
+    gw_params_t of the library, default: the reference defaults)."""
     sys.path.insert(0, ROOT)
-    from oracle.oracle import OracleModel, lib
+    from oracle.oracle import GwParams, OracleModel, lib
 
     L = lib("omp")
     gomp = C.CDLL("libgomp.so.1")
     all_threads = L.gwo_model_threads()
     gomp.omp_set_num_threads(int(threads or all_threads))
     threads = L.gwo_model_threads()
-    m = OracleModel(crus, seed=seed, kind="omp")
+    p = GwParams.from_buffer_copy(bytes(params)) if params is not None else None
+    m = OracleModel(crus, params=p, seed=seed, kind="omp")
     m.sample_rest_state()
     ist = np.zeros(steps + warmup_steps)
     t = 0.0
     for k in range(len(ist)):
-        ist[k] = 100.0 if t < 1.0 else 0.0
+        ist[k] = 100.0 if (t % period) < 1.0 else 0.0
         t += 1e-3
     if warmup_steps:
         m.run(1e-3, warmup_steps, ist[:warmup_steps])
@@ -237,13 +246,18 @@ def copy_calibration(nbytes, reps=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def advance_steps(args):
+    """Untimed steps before the timed window: to --window-start-ms, or --warmup if that is further."""
+    return max(int(args.warmup), int(round(args.window_start_ms / 1e-3)))
+
+
 def measure(capi, precision, args, total_crus):
     """One form (FP64 or FP32-local) on the bench batch: throughput and the SSA kernel's roofline."""
     f32 = precision == capi.PREC_FP32_LOCAL
     cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
     fixed = args.launch_fp32 if f32 else args.launch_fp64
     launch = tuple(int(x) for x in (fixed + "x1").split("x")[:3]) if fixed and args.autotune <= 0 else None
-    ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, cell_list, args.steps, args.warmup,
+    ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, cell_list, args.steps, advance_steps(args),
                                                           precision=precision, autotune=args.autotune, launch=launch)
     sim_ms = args.steps * 1e-3
     sec = ms / 1e3
@@ -284,15 +298,19 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000, help="timed Euler steps (dt = 1e-3 ms)")
-    ap.add_argument("--warmup", type=int, default=1000, help="untimed Euler steps before the timed region")
+    ap.add_argument("--warmup", type=int, default=1000,
+                    help="untimed Euler steps before the timed region (at least up to --window-start-ms)")
+    ap.add_argument("--window-start-ms", type=float, default=1.0,
+                    help="simulated time at which the timed window starts (default 1 ms: upstroke and release)")
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--crus", type=int, default=12500)
     ap.add_argument("--cpu-crus", type=int, default=12500)
-    ap.add_argument("--cpu-steps", type=int, default=40000)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="steps of the CPU sample (0: the timed window's)")
     ap.add_argument("--cpu-crus-1t", type=int, default=1500, help="CRUs of the one-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="seeds64",
-                    help="seeds64 (default): the seed ensemble; batch256 / large1024: the uneven and the large batch")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large1024",
+                    help="large1024 (default): 1,024 cells x 12,500 CRUs at 2 Hz; batch256: 256 uneven cells; "
+                         "seeds64: --cells independent seeds of one cell")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="run the autotuner with STEPS trial steps per launch shape, on a copy of the batch advanced "
                          "to the middle of the timed window, and time its choice instead of the fixed shapes")
@@ -316,12 +334,17 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
-    cell_list, _ = make_cells(capi, args.workload, args.cells, args.crus)
+    cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
     total_crus = sum(c.n_cru for c in cell_list)
     forms = {"fp32": capi.PREC_FP32_LOCAL, "fp64": capi.PREC_FP64}
     chosen = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    res = {k: measure(capi, forms[k], args, total_crus) for k in chosen}
+    res = {}
+    for k in chosen:
+        res[k] = measure(capi, forms[k], args, total_crus)
+        print(f"bench: {k} form timed, {res[k]['ms_per_step']:.4f} ms per step", file=sys.stderr, flush=True)
     head = res[chosen[0]]
+    t0_ms = advance_steps(args) * 1e-3
+    t1_ms = t0_ms + args.steps * 1e-3
     line = {
         "metric": "CRU\u00b7ms simulated/s and cell-beats/s, batched FP32-local vs plain FP64, 1 GPU",
         "value": head["value"],
@@ -337,11 +360,11 @@ def main(argv=None):
         "data": "synthetic: seeded rest state, paced from rest",
         "config": {"workload": (f"{args.cells} cells (independent seeds) x {args.crus} CRUs, 1 Hz pacing"
                                 if args.workload == "seeds64" else WORKLOADS[args.workload])
-                               + f", dt 1e-3 ms, timed window t = {args.warmup * 1e-3:g}.."
-                               f"{(args.warmup + args.steps) * 1e-3:g} ms of beat 1; "
+                               + f", dt 1e-3 ms, timed window t = {t0_ms:g}..{t1_ms:g} ms of beat 1; "
                                + ("batched FP32-local, plain FP64 beside it" if len(chosen) == 2 else chosen[0]),
                    "name": args.workload, "cells": len(cell_list), "crus": total_crus, "dt_ms": 1e-3,
-                   "precision": chosen[0]},
+                   "window_start_ms": t0_ms, "window_end_ms": t1_ms, "beat_ms": period,
+                   "untimed_steps_before_window": advance_steps(args), "precision": chosen[0]},
         "cell_beats_per_s": head["cell_beats_per_s"],
         "iterations_per_cru_step": head["iterations_per_cru_step"],
         "launch": head["launch"],
@@ -354,14 +377,20 @@ def main(argv=None):
         line["speedup_fp32_local_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     value = head["value"]
     if not args.no_cpu and rank == 0:
-        v, threads, secs = cpu_baseline(args.cpu_crus, args.cpu_steps)
-        v1, _, secs1 = cpu_baseline(args.cpu_crus_1t, args.cpu_steps, threads=1)
+        # the batch's first cell (its parameters, seed and pacing) over the same window
+        c0 = cell_list[0]
+        cpu_steps = args.cpu_steps or args.steps
+        adv = advance_steps(args)
+        v, threads, secs = cpu_baseline(args.cpu_crus, cpu_steps, adv, seed=c0.seed, params=c0.params, period=period)
+        v1, _, secs1 = cpu_baseline(args.cpu_crus_1t, cpu_steps, adv, seed=c0.seed, threads=1, params=c0.params,
+                                    period=period)
+        window = f"t = {adv * 1e-3:g}..{(adv + cpu_steps) * 1e-3:g} ms of beat 1 after {adv} untimed steps"
         line["cpu_baseline"] = {"value": v, "unit": "CRU\u00b7ms/s", "cores": threads, "kind": "port",
-                                "sample": f"1 cell x {args.cpu_crus} CRUs x {args.cpu_steps} steps "
-                                          f"(t = 0..{args.cpu_steps * 1e-3:g} ms of beat 1), {secs:.1f} s",
+                                "sample": f"the batch's first cell: 1 cell x {args.cpu_crus} CRUs x {cpu_steps} "
+                                          f"steps ({window}), {secs:.1f} s",
                                 "cpu_model": cpu_model(),
                                 "one_thread": {"value": v1, "sample": f"1 cell x {args.cpu_crus_1t} CRUs x "
-                                                                     f"{args.cpu_steps} steps, {secs1:.1f} s"},
+                                                                     f"{cpu_steps} steps ({window}), {secs1:.1f} s"},
                                 "ratio_gpu_over_cpu": value / v,
                                 "ratio_fp64_gpu_over_cpu": res["fp64"]["value"] / v if "fp64" in res else None}
     if rank == 0:

@@ -331,9 +331,11 @@ __global__ void k_init_cells(KArgs a)
         D.t = 0.0;
         D.step = 0;
         D.status = t.err ? GW_E_RANGE : 0;
+        D.stop_step = kNoStop;
         Currents cur;
         prepare_step(C, D, S, a.dt, cur, 0);
         lookahead_consts(C, D, a.dt, a.frozen != 0);
+        if (D.status != 0) D.stop_step = 0;
         D.obs.v_prev = D.g.V;
     }
 }
@@ -384,7 +386,7 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     acc_zero(acc);
     __syncthreads();
     const int64_t i = (b - C.blk0[kShape]) * (BLOCK / 4) + (tid >> 2);
-    const bool active = (i < C.n_cru) && (D.status == 0);
+    const bool active = (i < C.n_cru) && (step0 < D.stop_step);
     if (active) {
         R *const JLCC = LS::jlcc(a);
         const int64_t slot = C.slot0 + i;
@@ -501,7 +503,10 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CloseShared
     if (tid == 0 && a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
     if (status != 0) return;
     if (tot.err) {
-        if (tid == 0) D.status = GW_E_RANGE;
+        if (tid == 0) {
+            D.status = GW_E_RANGE;
+            D.stop_step = step_new + 1;   // SSA(step_new) runs beside this close; none after it
+        }
         return;
     }
     CloseLds &L = sh.L;
@@ -568,6 +573,11 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CloseShared
     if (!(g.V == g.V) || !(g.Cai == g.Cai)) {
         if (D.status == 0) D.status = GW_E_NAN;
     }
+    // A cell that fails here stops after the SSA of step_new, which runs beside
+    // this close in the same launch: the SSA blocks read stop_step, which only
+    // changes from a value above step_new to step_new + 1, so every block of
+    // that launch runs the step and none runs a later one.
+    if (D.status != 0) D.stop_step = step_new + 1;
     const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
     obs_update(D.obs, V_old, g.V, t_old, a.dt, g.Cai, cur.ICaL, open_frac, jrel * C.c.CRU_factor);
     if (a.trace_every > 0 && (step_new % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
@@ -637,7 +647,7 @@ __device__ __forceinline__ void ssa_block_refill(const KArgs &a, int b, int64_t 
     const int64_t w1 = min(w0 + kQuads * CPT, static_cast<int64_t>(C.n_cru));
     int64_t i = w0 + qd;
     int64_t next = w0 + kQuads;   // first CRU of the range not yet taken (wave-uniform)
-    bool have = (i < w1) && (D.status == 0);
+    bool have = (i < w1) && (step0 < D.stop_step);
     R *const JLCC = LS::jlcc(a);
     const R dt = static_cast<R>(a.dt);
     const uint32_t step_lo = static_cast<uint32_t>(step0 & 0xffff);
@@ -866,7 +876,7 @@ __device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t st
     const int64_t w1 = min(w0 + kWave * CPT, static_cast<int64_t>(C.n_cru));
     int64_t i = w0 + lane;
     int64_t next = w0 + kWave;   // first CRU of the range not yet taken (wave-uniform)
-    bool have = (i < w1) && (D.status == 0);
+    bool have = (i < w1) && (step0 < D.stop_step);
     LaneState S;
     if (have) {
         lane_load(a, C.slot0 + i, S);
@@ -961,6 +971,7 @@ __global__ void k_frozen_close(KArgs a, int64_t step0, int64_t nsteps)
     D.sums[3] = static_cast<double>(tot.clch); D.sums[4] = static_cast<double>(tot.open); D.sums[5] = fx_to_double(tot.jrel);
     for (int64_t s = 0; s < nsteps; ++s) D.t += a.dt;
     D.step += nsteps;
+    if (D.status != 0 && D.stop_step > D.step) D.stop_step = D.step;
 }
 
 // Packed device state <-> row-major host layout of one cell (GWCRUState, src/pyGW.cpp:78-87).
@@ -1050,6 +1061,65 @@ __global__ void k_reset_obs(CellDyn *dyn, int ncells)
     o.v_prev = V; o.apd_v_rest = V; o.apd_v_peak = V; o.apd_t_up = -1.0;
     o.apd90_last = -1.0; o.apd90_prev = -1.0; o.alt_sum = 0; o.alt_count = 0; o.beats_done = 0;
 }
+
+// Status and trace row count of every cell into out[0..n) and out[n..2n), so
+// that the host reads them with one copy.
+__global__ void k_gather_dyn(const CellDyn *dyn, int ncells, int64_t *out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    out[c] = dyn[c].status;
+    out[ncells + c] = dyn[c].trace_rows;
+}
+
+__global__ void k_zero_trace_rows(CellDyn *dyn, int ncells)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncells) dyn[c].trace_rows = 0;
+}
+
+// One cell whose globals, CRU state or stimulus changed after it has been
+// stepped: the derivatives and SSA constants of its current step are formed
+// again from its state (prepare_step + lookahead_consts, as the constructor
+// does at step 0), keeping its clock, step index and observables; no other
+// cell is touched. With resum != 0 (a new CRU state) the sums read from the
+// CRU state -- JLCC, ClCh, open RyRs -- are formed again from it, while Jxfer
+// and Jtr keep the values of the cell's last SSA: in the reference those are
+// per-CRU arrays only the SSA writes (GW.tpp:47-62), not part of GWCRUState.
+__global__ void k_refresh_cell(KArgs a, int cell, int resum)
+{
+    __shared__ Partial red[kBlock];
+    const DevCell &C = a.cells[cell];
+    CellDyn &D = a.dyn[cell];
+    if (resum) {
+        Partial mine;
+        partial_zero(mine);
+        for (int64_t i = threadIdx.x; i < C.n_cru; i += kBlock) {
+            const int64_t slot = C.slot0 + i;
+            double jl[4];
+            for (int j = 0; j < 4; ++j) jl[j] = ld_jlcc(a, slot * 4 + j);
+            mine.err |= fx_add(mine.jlcc, jl[0] + jl[1] + jl[2] + jl[3]);
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t w = a.sub[slot * 4 + j];
+                mine.clch += sub_cl(w);
+                mine.open += ryr_get(w, 2) + ryr_get(w, 3);
+            }
+        }
+        block_reduce<kBlock>(red, mine);
+    }
+    if (threadIdx.x != 0) return;
+    if (resum) {
+        D.sums[0] = fx_to_double(red[0].jlcc);
+        D.sums[3] = static_cast<double>(red[0].clch);
+        D.sums[4] = static_cast<double>(red[0].open);
+        if (red[0].err && D.status == 0) D.status = GW_E_RANGE;
+    }
+    const FluxSums S{D.sums[0], D.sums[1], D.sums[2], D.sums[3]};
+    Currents cur;
+    prepare_step(C, D, S, a.dt, cur, static_cast<int>(D.step & 1));
+    lookahead_consts(C, D, a.dt, a.frozen != 0);
+    if (D.status != 0 && D.stop_step > D.step) D.stop_step = D.step;
+}
 // ===========================================================================
 // host side
 // ===========================================================================
@@ -1138,6 +1208,7 @@ struct gw_batch {
     int32_t *d_block_cell = nullptr;
     void *d_staging = nullptr;
     size_t staging_bytes = 0;
+    int64_t *d_gather = nullptr;   // [2][ncells]: status, trace rows (k_gather_dyn)
 };
 
 namespace {
@@ -1456,12 +1527,34 @@ int launch_steps_timed(gw_batch *B, int64_t step0, int64_t n, double *ssa_ms, ch
     return rc;
 }
 
+// Status and trace row count of every cell, read with one copy.
+int gather_dyn(gw_batch *B, std::vector<int64_t> &out, char *err, size_t errlen)
+{
+    out.assign(2 * static_cast<size_t>(B->ncells), 0);
+    hipLaunchKernelGGL(k_gather_dyn, dim3((B->ncells + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, B->ncells,
+                       B->d_gather);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out.data(), B->d_gather, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
+// The whole per-cell dynamic state, read with one copy.
+int read_dyn(gw_batch *B, std::vector<CellDyn> &out, char *err, size_t errlen)
+{
+    out.resize(static_cast<size_t>(B->ncells));
+    HIP_TRY(hipMemcpyAsync(out.data(), B->d_dyn, out.size() * sizeof(CellDyn), hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    return GW_OK;
+}
+
 int check_status(gw_batch *B, char *err, size_t errlen)
 {
+    std::vector<int64_t> v;
+    const int rc = gather_dyn(B, v, err, errlen);
+    if (rc) return rc;
     for (int c = 0; c < B->ncells; ++c) {
-        int32_t st = 0;
-        HIP_TRY(hipMemcpyAsync(&st, &B->d_dyn[c].status, sizeof(int32_t), hipMemcpyDeviceToHost, B->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+        const int32_t st = static_cast<int32_t>(v[c]);
         if (st != 0) {
             const char *what = st == GW_E_NONPOSITIVE_CONC ? "a Nernst potential received a non-positive concentration"
                              : st == GW_E_NAN ? "a global state variable became NaN"
@@ -1471,6 +1564,21 @@ int check_status(gw_batch *B, char *err, size_t errlen)
             return st;
         }
     }
+    return GW_OK;
+}
+
+// After a per-cell override: before the first step the whole batch is
+// prepared afresh at step 0 (nothing has run); after it only this cell's
+// current step is formed again (k_refresh_cell).
+int refresh_cell(gw_batch *B, int cell, int resum, char *err, size_t errlen)
+{
+    if (!B->prepared || B->steps_done == 0) {
+        B->prepared = false;
+        return GW_OK;
+    }
+    hipLaunchKernelGGL(k_refresh_cell, dim3(1), dim3(kBlock), 0, B->ctx->stream, B->a, cell, resum);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     return GW_OK;
 }
 
@@ -1634,7 +1742,8 @@ int gw_batch_destroy(gw_batch *B)
     (void)hipStreamSynchronize(B->ctx->stream);
     graph_release(B);
     void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
-                    B->a.trace, B->a.iters_total, B->d_step_base, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging};
+                    B->a.trace, B->a.iters_total, B->d_step_base, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging,
+                    B->d_gather};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : B->istim_dev)
@@ -1732,6 +1841,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
         (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->max_nblocks, err, errlen)) ||
         (rc = dmalloc(&a.iters_total, 1, err, errlen)) || (rc = dmalloc(&B->d_step_base, 1, err, errlen)) ||
         (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
+        (rc = dmalloc(&B->d_gather, 2 * static_cast<size_t>(ncells), err, errlen)) ||
         (rc = dmalloc(&B->d_block_cell, B->block_cell_h.size(), err, errlen))) {
         gw_batch_destroy(B);
         return rc;
@@ -1828,8 +1938,7 @@ int gw_batch_set_cru_state(gw_batch *B, int32_t cell, const gw_cru_state_t *c, c
     hipLaunchKernelGGL(k_put_state, dim3((n + 255) / 256), dim3(256), 0, s, B->a, cell, v, c->JLCC ? 1 : 0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
-    B->prepared = false;
-    return GW_OK;
+    return refresh_cell(B, cell, 1, err, errlen);
 }
 
 int gw_batch_get_cru_state(gw_batch *B, int32_t cell, gw_cru_state_t *c, char *err, size_t errlen)
@@ -1890,8 +1999,7 @@ int gw_batch_set_globals(gw_batch *B, int32_t cell, const gw_globals_t *g, char 
     HIP_TRY(hipSetDevice(B->ctx->device));
     HIP_TRY(hipMemcpyAsync(&B->d_dyn[cell].g, g, sizeof(gw_globals_t), hipMemcpyHostToDevice, B->ctx->stream));
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
-    B->prepared = false;
-    return GW_OK;
+    return refresh_cell(B, cell, 0, err, errlen);
 }
 
 int gw_batch_get_globals(gw_batch *B, gw_globals_t *g_out, char *err, size_t errlen)
@@ -1901,9 +2009,10 @@ int gw_batch_get_globals(gw_batch *B, gw_globals_t *g_out, char *err, size_t err
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(B->ctx->device));
-    for (int c = 0; c < B->ncells; ++c)
-        HIP_TRY(hipMemcpyAsync(&g_out[c], &B->d_dyn[c].g, sizeof(gw_globals_t), hipMemcpyDeviceToHost, B->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    std::vector<CellDyn> d;
+    const int rc = read_dyn(B, d, err, errlen);
+    if (rc) return rc;
+    for (int c = 0; c < B->ncells; ++c) g_out[c] = d[c].g;
     return GW_OK;
 }
 
@@ -1911,6 +2020,11 @@ int gw_batch_set_stimulus_table(gw_batch *B, int32_t cell, const double *istim, 
 {
     if (!B || cell < 0 || cell >= B->ncells || (n > 0 && !istim) || n < 0) {
         set_err(err, errlen, "gw_batch_set_stimulus_table: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    if (B->cells_h[cell].proto.kind == GW_PROTO_CLAMP) {
+        set_err(err, errlen, "gw_batch_set_stimulus_table: cell %d is voltage-clamped; a stimulus table has no effect on it",
+                cell);
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(B->ctx->device));
@@ -1927,8 +2041,7 @@ int gw_batch_set_stimulus_table(gw_batch *B, int32_t cell, const double *istim, 
     B->cells_h[cell].proto.kind = GW_PROTO_TABLE;
     HIP_TRY(hipMemcpyAsync(&B->d_cells[cell], &B->cells_h[cell], sizeof(DevCell), hipMemcpyHostToDevice, B->ctx->stream));
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
-    B->prepared = false;
-    return GW_OK;
+    return refresh_cell(B, cell, 0, err, errlen);
 }
 
 int gw_batch_step(gw_batch *B, int64_t nsteps, char *err, size_t errlen)
@@ -1977,13 +2090,19 @@ int gw_batch_get_time(gw_batch *B, double *t, int64_t *steps_done)
         if (steps_done) *steps_done = 0;
         return GW_OK;
     }
-    double tt = 0.0;
-    int64_t st = 0;
-    (void)hipMemcpyAsync(&tt, &B->d_dyn[0].t, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream);
-    (void)hipMemcpyAsync(&st, &B->d_dyn[0].step, sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream);
-    if (hipStreamSynchronize(B->ctx->stream) != hipSuccess) return GW_E_HIP;
-    if (t) *t = tt;
-    if (steps_done) *steps_done = st;
+    // every running cell shares the clock; a cell that failed stopped its own
+    std::vector<CellDyn> d;
+    if (read_dyn(B, d, nullptr, 0) != GW_OK) return GW_E_HIP;
+    int pick = -1;
+    for (int c = 0; c < B->ncells && pick < 0; ++c)
+        if (d[c].status == 0) pick = c;
+    if (pick < 0) {
+        pick = 0;
+        for (int c = 1; c < B->ncells; ++c)
+            if (d[c].step > d[pick].step) pick = c;
+    }
+    if (t) *t = d[pick].t;
+    if (steps_done) *steps_done = d[pick].step;
     return GW_OK;
 }
 
@@ -1994,9 +2113,10 @@ int gw_batch_get_obs(gw_batch *B, gw_cell_obs_t *obs, char *err, size_t errlen)
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(B->ctx->device));
-    for (int c = 0; c < B->ncells; ++c)
-        HIP_TRY(hipMemcpyAsync(&obs[c], &B->d_dyn[c].obs, sizeof(gw_cell_obs_t), hipMemcpyDeviceToHost, B->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    std::vector<CellDyn> d;
+    const int rc = read_dyn(B, d, err, errlen);
+    if (rc) return rc;
+    for (int c = 0; c < B->ncells; ++c) obs[c] = d[c].obs;
     return GW_OK;
 }
 
@@ -2029,7 +2149,8 @@ int gw_batch_enable_trace(gw_batch *B, int32_t every, int64_t capacity, char *er
         B->a.trace_every = every;
         B->a.trace_cap = capacity;
     }
-    for (int c = 0; c < B->ncells; ++c) HIP_TRY(hipMemsetAsync(&B->d_dyn[c].trace_rows, 0, sizeof(int64_t), B->ctx->stream));
+    hipLaunchKernelGGL(k_zero_trace_rows, dim3((B->ncells + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, B->ncells);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     return GW_OK;
 }
@@ -2041,13 +2162,23 @@ int gw_batch_get_trace(gw_batch *B, double *out, int64_t max_rows, int64_t *rows
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(B->ctx->device));
+    std::vector<int64_t> v;
+    int rc = gather_dyn(B, v, err, errlen);
+    if (rc) return rc;
+    const int n = B->ncells;
     int64_t r = 0;
-    HIP_TRY(hipMemcpyAsync(&r, &B->d_dyn[0].trace_rows, sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    for (int c = 0; c < n; ++c) r = std::max(r, v[n + c]);
     r = std::min(r, max_rows);
-    if (r > 0 && out)
-        HIP_TRY(hipMemcpyAsync(out, B->a.trace, static_cast<size_t>(r) * B->ncells * 4 * sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
-    for (int c = 0; c < B->ncells; ++c) HIP_TRY(hipMemsetAsync(&B->d_dyn[c].trace_rows, 0, sizeof(int64_t), B->ctx->stream));
+    if (r > 0 && out) {
+        HIP_TRY(hipMemcpyAsync(out, B->a.trace, static_cast<size_t>(r) * n * 4 * sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+        // a cell that stopped early (non-zero status) wrote fewer rows: its later rows read NaN
+        for (int c = 0; c < n; ++c)
+            for (int64_t k = std::max<int64_t>(0, v[n + c]); k < r; ++k)
+                for (int q = 0; q < 4; ++q) out[(k * n + c) * 4 + q] = std::nan("");
+    }
+    hipLaunchKernelGGL(k_zero_trace_rows, dim3((n + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, n);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     *rows = r;
     return GW_OK;

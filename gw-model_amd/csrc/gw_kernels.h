@@ -8,6 +8,7 @@ namespace gw {
 constexpr int kBlock = 256;        // threads per block of the setup kernels (64 CRUs)
 constexpr int kWave = 64;
 constexpr int kCruPerBlock = kBlock / 4;  // one quad of lanes per CRU
+constexpr int64_t kNoStop = 0x7fffffffffffffffLL;   // CellDyn::stop_step of a cell that has not failed
 
 // Block -> cell maps of the SSA launch shapes: map s gives blocks of 16 << s
 // CRUs (16..2048). A quad kernel with T threads uses map log2(T / 64); a lane
@@ -50,6 +51,7 @@ struct CellDyn {
     double sums[6];      // JLCC, Jxfer, Jtr, ClCh, open RyR, Jrel of the last SSA
     int64_t step;        // index of the current step
     int64_t trace_rows;
+    int64_t stop_step;   // SSA steps from this index on are not run (set when the cell fails; kNoStop)
     int32_t status;
     int32_t clamp;
     gw_cell_obs_t obs;

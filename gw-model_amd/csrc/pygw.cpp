@@ -220,11 +220,20 @@ RunInput run_input(int nCRU, double step_size, int num_steps, const py::object &
     return in;
 }
 
+// The precision keyword of run / run_reduced (additive; the reference runs FP64 only).
+int32_t precision_from_name(const std::string &name)
+{
+    if (name == "fp64") return GW_PREC_FP64;
+    if (name == "fp32_local") return GW_PREC_FP32_LOCAL;
+    throw py::value_error("precision must be \"fp64\" or \"fp32_local\" (got \"" + name + "\")");
+}
+
 // run_PRNG_arg + run<PRNG> (src/pyGW.cpp:134-157, src/pyGW.hpp:102-122)
 PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int num_steps, const py::object &Istim,
                    int record_every, PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng,
-                   py::object seed)
+                   py::object seed, const std::string &precision)
 {
+    const int32_t prec = precision_from_name(precision);
     RunInput in = run_input(nCRU, step_size, num_steps, Istim, record_every, init_crus, init_globals, rng, seed);
     const int R = static_cast<int>(in.records);
     PyGWSimulation out(nCRU, R, num_steps * step_size);
@@ -245,7 +254,7 @@ PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int nu
         std::lock_guard<std::mutex> lock(g_mutex);
         gw_ctx *ctx = context();
         rc = gw_run(ctx, &params, nCRU, step_size, num_steps, in.istim.data(), record_every, &in.g0, &in.c0, in.seed,
-                    in.prng, GW_PREC_FP64, &rec, err, sizeof(err));
+                    in.prng, prec, &rec, err, sizeof(err));
     }
     if (rc != GW_OK) raise_status(rc, err);
     return out;
@@ -256,8 +265,9 @@ PyGWSimulation run(const gw_params_t &params, int nCRU, double step_size, int nu
 // keys, dtypes and shapes as sim_dict_reduced_memory(GWSolution(run(...))).
 py::dict run_reduced(const gw_params_t &params, int nCRU, double step_size, int num_steps, const py::object &Istim,
                      int record_every, PyCRUState &init_crus, PyGlobalState &init_globals, const std::string &rng,
-                     py::object seed)
+                     py::object seed, const std::string &precision)
 {
+    const int32_t prec = precision_from_name(precision);
     RunInput in = run_input(nCRU, step_size, num_steps, Istim, record_every, init_crus, init_globals, rng, seed);
     const py::ssize_t R = in.records, N = nCRU;
     py::dict out;
@@ -291,7 +301,7 @@ py::dict run_reduced(const gw_params_t &params, int nCRU, double step_size, int 
         std::lock_guard<std::mutex> lock(g_mutex);
         gw_ctx *ctx = context();
         rc = gw_run_reduced(ctx, &params, nCRU, step_size, num_steps, in.istim.data(), record_every, &in.g0, &in.c0,
-                            in.seed, in.prng, GW_PREC_FP64, &rec, err, sizeof(err));
+                            in.seed, in.prng, prec, &rec, err, sizeof(err));
     }
     if (rc != GW_OK) raise_status(rc, err);
     return out;
@@ -381,12 +391,12 @@ PYBIND11_MODULE(GreensteinWinslow, m)
 
     m.def("run", &run, "Simulate the model", "parameters"_a, "nCRU"_a, "step_size"_a, "num_steps"_a, "Istim"_a,
           "record_every"_a, "init_crus"_a, "init_globals"_a = PyGlobalState(), "PRNG"_a = "mt19937_64",
-          "seed"_a = py::none());
+          "seed"_a = py::none(), "precision"_a = "fp64");
 
     m.def("run_reduced", &run_reduced,
           "run() with its records in the reduced encoding of sim_dict_reduced_memory, formed on the device",
           "parameters"_a, "nCRU"_a, "step_size"_a, "num_steps"_a, "Istim"_a, "record_every"_a, "init_crus"_a,
-          "init_globals"_a = PyGlobalState(), "PRNG"_a = "mt19937_64", "seed"_a = py::none());
+          "init_globals"_a = PyGlobalState(), "PRNG"_a = "mt19937_64", "seed"_a = py::none(), "precision"_a = "fp64");
 
     m.def("rest_state", [](int nCRU, uint64_t seed, const std::string &rng) {
         PyCRUState s(nCRU);

@@ -146,11 +146,14 @@ class GWModel:
             self.__cru_state = gw_cxx.GWCRUState(self.parameters.NCaRU_sim)
 
     def simulate(self, step_size: float, num_steps: int, record_every: int = 1, PRNG: str = "xoshiro256++",
-                 seed: int | None = None) -> GWSolution:
+                 seed: int | None = None, precision: str = "fp64") -> GWSolution:
         """Simulate num_steps Euler steps of size step_size [ms], recording every record_every steps.
 
         seed (additive): seeds the per-CRU streams for a reproducible run; None
         draws a seed from std::random_device like the reference.
+        precision (additive): "fp64" (the reference's arithmetic, the default) or
+        "fp32_local" (CRU-local Ca and propensities in FP32; globals, channel
+        states and flux sums as in fp64).
         """
         assert_positive(step_size, "step_size")
         assert_positive(num_steps, "num_steps")
@@ -158,7 +161,7 @@ class GWModel:
         try:
             sol = gw_cxx.run(self.parameters.cxx_struct, self.parameters.NCaRU_sim, step_size, num_steps, self.__stim,
                              record_every, init_crus=self.__cru_state, init_globals=self.__global_state, PRNG=PRNG,
-                             seed=seed)
+                             seed=seed, precision=precision)
         except ValueError as e:
             if PRNG not in IMPLEMENTED_PRNGS:
                 raise ValueError(f"{PRNG} is an invalid argument for PRNG.\nAvailable options are: "
@@ -167,7 +170,7 @@ class GWModel:
         return GWSolution(sol, self.parameters)
 
     def simulate_reduced(self, step_size: float, num_steps: int, record_every: int = 1, PRNG: str = "xoshiro256++",
-                         seed: int | None = None) -> dict:
+                         seed: int | None = None, precision: str = "fp64") -> dict:
         """simulate() with its records returned in the reduced encoding of
         sim_dict_reduced_memory (src/serialization.py:11-39), formed on the
         device: the same dictionary as sim_dict_reduced_memory(simulate(...))
@@ -180,7 +183,7 @@ class GWModel:
                              + ", ".join(self.PRNG_options()))
         return gw_cxx.run_reduced(self.parameters.cxx_struct, self.parameters.NCaRU_sim, step_size, num_steps,
                                   self.__stim, record_every, init_crus=self.__cru_state,
-                                  init_globals=self.__global_state, PRNG=PRNG, seed=seed)
+                                  init_globals=self.__global_state, PRNG=PRNG, seed=seed, precision=precision)
 
     def stimulus_fn(self, t: float) -> float:
         return self.__stim(t)

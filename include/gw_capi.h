@@ -282,6 +282,16 @@ int gw_run_reduced(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt,
  * If g0 is NULL the reference default globals are used for every cell.
  * CRU state starts at the seeded rest-state sampler (src/includes/GW_utils.tpp:5-64);
  * gw_batch_set_cru_state overrides it per cell.
+ *
+ * Per-cell overrides (gw_batch_set_cru_state, gw_batch_set_globals,
+ * gw_batch_set_stimulus_table) touch only the named cell. Before the first
+ * gw_batch_step they set that cell's starting point (step 0, as the
+ * reference's constructor and set_initial_value, GW.hpp:93-113, GW.tpp:5-10).
+ * After it the cell keeps its clock, step index and observables, and its
+ * current step's derivatives and SSA constants are formed again from the new
+ * state; a new CRU state also re-forms the JLCC and ClCh sums, while Jxfer and
+ * Jtr keep the values of the cell's last SSA (in the reference they are
+ * per-CRU arrays written only by the SSA, GW.tpp:47-62). No other cell changes.
  */
 int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells,
                     int32_t precision, int32_t prng_id, double dt,
@@ -298,7 +308,9 @@ int gw_batch_set_globals(gw_batch *b, int32_t cell, const gw_globals_t *g,
                          char *err, size_t errlen);
 int gw_batch_get_globals(gw_batch *b, gw_globals_t *g_out /* [ncells] */,
                          char *err, size_t errlen);
-/* Per-step table stimulus for one cell (GW_PROTO_TABLE); `istim` has n entries, copied to the device. */
+/* Per-step table stimulus for one cell (GW_PROTO_TABLE); `istim` has n entries, copied to the device.
+ * Entry k is the stimulus of step k; steps past the table get 0. A voltage-clamped cell
+ * (GW_PROTO_CLAMP) is refused with GW_E_INVALID_ARG: a stimulus has no effect under a clamp. */
 int gw_batch_set_stimulus_table(gw_batch *b, int32_t cell, const double *istim, int64_t n,
                                 char *err, size_t errlen);
 /*
@@ -310,14 +322,17 @@ int gw_batch_set_stimulus_table(gw_batch *b, int32_t cell, const double *istim, 
 int gw_batch_set_graph(gw_batch *b, int32_t steps_per_graph, char *err, size_t errlen);
 /* Advance every cell by nsteps Euler steps. */
 int gw_batch_step(gw_batch *b, int64_t nsteps, char *err, size_t errlen);
-/* Simulated time of the batch (accumulated t += dt, as src/pyGW.hpp:116). */
+/* Simulated time of the batch (accumulated t += dt, as src/pyGW.hpp:116): the clock of the
+ * running cells, which they share; if every cell has stopped on an error, the latest clock. */
 int gw_batch_get_time(gw_batch *b, double *t, int64_t *steps_done);
 int gw_batch_get_obs(gw_batch *b, gw_cell_obs_t *obs /* [ncells] */, char *err, size_t errlen);
 int gw_batch_reset_obs(gw_batch *b, char *err, size_t errlen);
 /*
  * Reduced trace: every `every` steps, per cell, (V, Cai, ICaL, open RyR fraction)
  * is appended to a device ring of `capacity` rows; gw_batch_get_trace copies out
- * rows [0, *rows) as [rows][ncells][4] doubles and clears the ring.
+ * rows [0, *rows) as [rows][ncells][4] doubles and clears the ring. *rows is the
+ * largest row count over the cells: a cell that stopped on an error wrote fewer
+ * rows, and its rows after the stop read NaN.
  */
 int gw_batch_enable_trace(gw_batch *b, int32_t every, int64_t capacity, char *err, size_t errlen);
 int gw_batch_get_trace(gw_batch *b, double *out, int64_t max_rows, int64_t *rows,

@@ -71,3 +71,29 @@ def _parser_actions(bench):
     finally:
         argparse.ArgumentParser.parse_args = real
     return captured["actions"]
+
+
+def test_timed_window_starts_at_the_stated_time(bench):
+    import argparse
+
+    ns = argparse.Namespace(warmup=5, window_start_ms=1.0)
+    assert bench.advance_steps(ns) == 1000          # a short --warmup still reaches the window
+    ns = argparse.Namespace(warmup=4000, window_start_ms=1.0)
+    assert bench.advance_steps(ns) == 4000          # a longer one moves it later
+
+
+def test_cpu_baseline_runs_the_batch_first_cell_over_the_window(bench):
+    """The CPU sample takes the first cell's parameters, seed and pacing period
+    and the same untimed advance as the GPU leg (child process: -Ofast build)."""
+    capi = bench.load_capi()
+    cells, period = bench.make_cells(capi, "large1024", 0, 0)
+    assert len(cells) == 1024 and period == 500.0
+    c0 = cells[0]
+    assert c0.params.PCaL != capi.Params.default().PCaL   # scaled conductances
+    code = ("import json, sys; sys.path.insert(0, sys.argv[1]); import bench; capi = bench.load_capi(); "
+            "cells, period = bench.make_cells(capi, 'large1024', 0, 0); c0 = cells[0]; "
+            "print(json.dumps(bench.cpu_baseline(32, 200, 100, seed=c0.seed, params=c0.params, period=period)))")
+    out = subprocess.run([sys.executable, "-c", code, ROOT], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    v, threads, secs = json.loads(out.stdout.strip().splitlines()[-1])
+    assert v > 0 and threads >= 1 and secs > 0

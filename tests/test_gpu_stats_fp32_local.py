@@ -4,8 +4,13 @@ CRU-local Ca, fluxes and propensities run in FP32; channel states, random
 streams, flux sums (exact, FP64) and the global ODE (FP64) are those of the
 FP64 form. The FP64 form is bit-identical to the CPU oracle (test_gpu_parity.py),
 so it stands for the reference here. The variant is not bit-exact, so its
-results are compared as ensembles: per-cell observables over independent seeds,
-means within 3 standard errors.
+results are compared as ensembles over seeds: each seed runs in both forms and
+the per-seed differences are tested (mean within 3 standard errors of the
+paired differences), the spreads are compared (variance ratio, two-sample
+Kolmogorov-Smirnov), and the relative error of each observable is printed with
+its 95% confidence interval. The one-lane layouts draw their uniforms
+differently from the FP64 quads (two per engine draw), so each seed's two runs
+are different sample paths and a bias or a wider spread of the variant shows.
 """
 import math
 
@@ -42,6 +47,7 @@ def _stats(obs):
 
 
 def _check_within_3se(a, b, name):
+    """Unpaired: mean of a within 3 SE of mean of b."""
     se = math.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
     diff = a.mean() - b.mean()
     say(f"  {name}: fp64 {b.mean():.6g} fp32 {a.mean():.6g} diff {diff:.3g} se {se:.3g} "
@@ -50,16 +56,49 @@ def _check_within_3se(a, b, name):
     return diff / b.mean()
 
 
+def _check_paired(a, b, name):
+    """Paired by seed: the mean of the per-seed differences a - b is within 3
+    of its standard errors of 0. Prints the relative error (mean difference
+    over the FP64 mean) with its 95% confidence interval."""
+    d = a - b
+    n = len(d)
+    se = d.std(ddof=1) / math.sqrt(n)
+    m = d.mean()
+    ref = b.mean()
+    say(f"  {name}: fp64 {ref:.6g} fp32 {a.mean():.6g} | paired diff {m:.3g} +- {1.96 * se:.2g} (95% CI) "
+        f"| rel error {m / ref:+.2e} +- {1.96 * se / abs(ref):.1e}")
+    assert abs(m) <= 3 * se + 1e-12 * abs(ref), name
+    return m / ref, 1.96 * se / abs(ref)
+
+
+def _check_spread(a, b, name, alpha=1e-3):
+    """The spread of a is that of b: variance ratio within the two-sided
+    1 - alpha interval of the F distribution, and a two-sample
+    Kolmogorov-Smirnov test not rejecting at level alpha."""
+    from scipy import stats
+
+    n = len(a)
+    ratio = a.var(ddof=1) / b.var(ddof=1)
+    lo, hi = stats.f.ppf(alpha / 2, n - 1, n - 1), stats.f.ppf(1 - alpha / 2, n - 1, n - 1)
+    ks = stats.ks_2samp(a, b)
+    say(f"  {name}: SD fp64 {b.std(ddof=1):.4g} fp32 {a.std(ddof=1):.4g} variance ratio {ratio:.3f} "
+        f"(accept {lo:.2f}..{hi:.2f}), KS p {ks.pvalue:.3g}")
+    assert lo <= ratio <= hi, name
+    assert ks.pvalue > alpha, name
+
+
 _FP64_ENSEMBLE = {}
 
 
-@pytest.mark.parametrize("launch", [(4, 256, 1), (1, 64, 1), (1, 64, 4)], ids=["quad", "lane", "lane-refill"])
+@pytest.mark.parametrize("launch", [(1, 64, 1), (1, 64, 4)], ids=["lane", "lane-refill"])
 def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx, launch):
-    """64 seeds, one paced beat (600 ms: past the longest APD90 of the FP64
-    form, about 420 ms over 256 seeds, and of the FP32-local form's longer
-    tail, about 500 ms, so no cell's beat is cut off): APD90, peak Cai, peak
-    ICaL and mean RyR open fraction agree within 3 SE, for a quad of lanes per
-    CRU, for one lane per CRU and for lanes refilled from four CRUs each."""
+    """64 seeds of a 1,000-CRU cell, one paced beat (600 ms: past the longest
+    APD90 of either form, so no cell's beat is cut off): APD90, peak Cai, peak
+    ICaL and mean RyR open fraction agree seed by seed within 3 SE, and the
+    spread of APD90 and peak Cai is the same, for one lane per CRU and for
+    lanes refilled from four CRUs each. (The quad layout takes the top 24 bits
+    of the FP64 form's own draws, so its runs track the FP64 ones seed by seed
+    and are checked for invariants only.)"""
     seeds, n, steps = 64, 1000, 600_000
     cells = [gwcapi.make_cell(n, 500 + s, None, PACE) for s in range(seeds)]
     if "o64" not in _FP64_ENSEMBLE:
@@ -75,11 +114,52 @@ def test_fp32_local_free_mode_ensemble_matches_fp64(gwcapi, gpu_ctx, launch):
         say(f"  {name}: cells without one full beat: {bad.tolist()} APD90 {a['APD90'][bad].tolist()} final V {vb}")
     assert np.all(a64["APD90"] > 100) and np.all(a32["APD90"] > 100)  # every cell repolarised once
     for k in a64:
-        _check_within_3se(a32[k], a64[k], k)
+        _check_paired(a32[k], a64[k], k)
+    for k in ("APD90", "peak_Cai"):
+        _check_spread(a32[k], a64[k], k)
     v32 = np.array([x.V for x in g32])
     say(f"  fp32 final V: min {v32.min():.2f} max {v32.max():.2f} mV")
     # every cell repolarised (APD90 above) and none has blown up since
     assert np.all(np.isfinite(v32)) and np.all(v32 < -60.0)
+
+
+def test_fp32_local_seed_ensemble_at_12500_crus(gwcapi, gpu_ctx):
+    """The 64-seed ensemble of a 12,500-CRU cell (the seed ensemble of the
+    plain-method configuration), one paced beat (600 ms), the bench's launch
+    shapes (FP64 quads 4 x 64, FP32-local lanes refilled from four CRUs,
+    1 x 64 x 4): per-seed paired errors of APD90, peak Cai, peak ICaL and mean
+    RyR open fraction within 3 SE, the same spread of APD90 and peak Cai, and
+    in the FP32-local form every channel count valid and every cell back at
+    rest."""
+    seeds, n, steps = 64, 12500, 600_000
+    cells = [gwcapi.make_cell(n, 42 + s, None, PACE) for s in range(seeds)]
+    say(f"fp64 ensemble {seeds} x {n} CRUs x {steps} steps")
+    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps, (4, 64, 1))
+    say("fp32-local ensemble, launch (1, 64, 4)")
+    b = gwcapi.Batch(gpu_ctx, cells, precision=gwcapi.PREC_FP32_LOCAL)
+    b.set_mode(frozen=False, digest=False)
+    b.set_launch(1, 64, 4)
+    for k in range(6):
+        b.step(steps // 6)
+        say(f"  fp32-local t = {(k + 1) * steps // 6 // 1000} ms")
+    o32 = b.obs()
+    for c in range(len(cells)):
+        R = b.get_cru_state(c)["RyR"]
+        assert np.all(R.sum(axis=2) == 5), c
+        assert np.all(R[..., 0] >= 0) and np.all(R[..., 1] >= 0), c
+        assert np.all(R[..., 2] + R[..., 3] >= 0) and np.all(R[..., 4] + R[..., 5] >= 0), c
+    g32 = b.get_globals()
+    b.close()
+    a64, a32 = _stats(o64), _stats(o32)
+    say(f"  APD90 fp64 {a64['APD90'].min():.1f}..{a64['APD90'].max():.1f} ms, "
+        f"fp32 {a32['APD90'].min():.1f}..{a32['APD90'].max():.1f} ms")
+    assert np.all(a64["APD90"] > 100) and np.all(a32["APD90"] > 100)
+    for k in a64:
+        _check_paired(a32[k], a64[k], k)
+    for k in ("APD90", "peak_Cai"):
+        _check_spread(a32[k], a64[k], k)
+    v32 = np.array([x.V for x in g32])
+    assert np.all(np.isfinite(v32)) and np.all(v32 < -80.0)
 
 
 @pytest.mark.parametrize("launch", [(4, 256, 1), (1, 64, 1), (1, 256, 1), (1, 128, 2), (1, 64, 4), (1, 256, 8)],
@@ -156,35 +236,14 @@ def test_fp32_local_channel_counts_stay_valid_at_scale(gwcapi, gpu_ctx, launch):
     b.close()
 
 
-def test_fp32_local_whole_beat_at_scale(gwcapi, gpu_ctx):
-    """64 cells x 12,500 CRUs through a whole paced beat (500 ms, about 30 s on
-    one GPU): every channel count stays valid, every cell repolarises with an
-    APD90 in the range of the FP64 ensemble, and the cells return to rest."""
-    cells = [gwcapi.make_cell(12500, 42 + s, None, PACE) for s in range(64)]
-    b = gwcapi.Batch(gpu_ctx, cells, precision=gwcapi.PREC_FP32_LOCAL)
-    b.set_mode(frozen=False, digest=False)
-    b.set_launch(1, 64, 4)
-    for k in range(5):
-        say(f"fp32-local whole beat at scale, t = {k * 100}..{(k + 1) * 100} ms")
-        b.step(100_000)
-    for c in range(len(cells)):
-        R = b.get_cru_state(c)["RyR"]
-        assert np.all(R.sum(axis=2) == 5)
-        assert np.all(R[..., 0] >= 0) and np.all(R[..., 1] >= 0)
-        assert np.all(R[..., 2] + R[..., 3] >= 0) and np.all(R[..., 4] + R[..., 5] >= 0)
-    apd = np.array([o["apd90_last"] for o in b.obs()])
-    v = np.array([x.V for x in b.get_globals()])
-    say(f"  APD90 {apd.min():.1f}..{apd.max():.1f} ms, final V {v.min():.2f}..{v.max():.2f} mV")
-    assert np.all((apd > 250) & (apd < 450))
-    assert np.all(v < -85.0)
-    b.close()
-
-
 def test_fp32_local_clamp_family_errors(gwcapi, gpu_ctx):
-    """Voltage-clamp steps from -80 mV to -40..+40 mV: peak ICaL and the
-    integrated RyR release flux of the FP32-local form agree with FP64 within
-    3 SE at every step potential."""
-    seeds, n, hold_ms, step_ms = 8, 2000, 5.0, 50.0
+    """The voltage-clamp family at full cell size: 12,500 CRUs held at -80 mV
+    for 10 ms, then stepped to -40..+40 mV in 10 mV increments for 200 ms,
+    4 seeds per potential in both forms. Peak ICaL and the integrated RyR
+    release flux of the FP32-local form agree with FP64 seed by seed within
+    3 SE at every step potential; the relative errors are printed with their
+    95% confidence intervals."""
+    seeds, n, hold_ms, step_ms = 4, 12500, 10.0, 200.0
     volts = [-40.0, -30.0, -20.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0]
     steps = int(round((hold_ms + step_ms) * 1000))
     cells = []
@@ -192,12 +251,12 @@ def test_fp32_local_clamp_family_errors(gwcapi, gpu_ctx):
         proto = {"kind": gwcapi.PROTO_CLAMP, "hold": -80.0, "step": v, "t_on": hold_ms - 5e-4,
                  "t_off": hold_ms + step_ms}
         cells += [gwcapi.make_cell(n, 900 + s, None, proto) for s in range(seeds)]
-    say(f"clamp family {len(volts)} x {seeds} seeds x {n} CRUs")
-    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps)
-    o32, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps)
+    say(f"clamp family {len(volts)} x {seeds} seeds x {n} CRUs x {steps} steps")
+    o64, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP64, cells, steps, (4, 64, 1))
+    o32, _, _ = _run(gwcapi, gpu_ctx, gwcapi.PREC_FP32_LOCAL, cells, steps, (1, 64, 4))
     for i, v in enumerate(volts):
         sl = slice(i * seeds, (i + 1) * seeds)
         for key in ("peak_ICaL_neg", "int_Jrel"):
             a = np.array([o[key] for o in o32[sl]])
             b = np.array([o[key] for o in o64[sl]])
-            _check_within_3se(a, b, f"{key} at {v:+.0f} mV")
+            _check_paired(a, b, f"{key} at {v:+.0f} mV")

@@ -141,6 +141,12 @@ def test_run_argument_errors(gw):
         gw.run(p, 4, 1e-3, 10, lambda t: 0.0, 0, c)
     with pytest.raises(ValueError):
         gw.run(p, 5, 1e-3, 10, lambda t: 0.0, 1, c)  # init_crus has 4 CRUs
+    # precision is an additive keyword: fp64 (default) or fp32_local; anything else is refused before any GPU call
+    for fn in (gw.run, gw.run_reduced):
+        with pytest.raises(ValueError, match="precision"):
+            fn(p, 4, 1e-3, 10, lambda t: 0.0, 1, c, precision="fp16")
+    doc = gw.run.__doc__
+    assert "precision" in doc and "fp64" in doc
 
 
 def test_parameters_wrapper(pkg):
