@@ -154,6 +154,10 @@ def lib() -> C.CDLL:
         "gw_batch_time_steps": (C.c_int, [P, i64, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)] + E),
         "gw_batch_get_launch": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "gw_batch_set_graph": (C.c_int, [P, i32] + E),
+        "gw_batch_set_persistent": (C.c_int, [P, i32] + E),
+        "gw_batch_get_persistent": (C.c_int, [P, C.POINTER(i32)]),
+        "gw_batch_set_spin_limit": (C.c_int, [P, i64] + E),
+        "gw_batch_get_currents": (C.c_int, [P, P, P] + E),
         "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                         C.POINTER(f64)] + E),
         "gw_run_batch": (C.c_int, [P, i32, P, i32, f64, i64, P, P] + E),
@@ -374,6 +378,27 @@ class Batch:
         """Replay step launches as captured HIP graphs of this many steps (0: plain launches)."""
         _call(self.L.gw_batch_set_graph, self.h, int(steps_per_graph))
 
+    def set_persistent(self, mode: int):
+        """Persistent stepping: 0 one launch per step, 1 one launch per call (refused if the
+        blocks cannot all be resident), 2 persistent when they can (default)."""
+        _call(self.L.gw_batch_set_persistent, self.h, int(mode))
+
+    def currents(self) -> tuple[np.ndarray, np.ndarray]:
+        """(ICaL, Ito2) of every cell by the post-hoc definitions of src/model.py:414-481, on the device."""
+        i, t = np.zeros(self.n), np.zeros(self.n)
+        _call(self.L.gw_batch_get_currents, self.h, _ptr(i), _ptr(t))
+        return i, t
+
+    def set_spin_limit(self, polls: int):
+        """Polls before a wait of the persistent kernel gives up (0: the default)."""
+        _call(self.L.gw_batch_set_spin_limit, self.h, int(polls))
+
+    def persistent(self) -> bool:
+        """Whether the next steps run in the persistent kernel."""
+        v = i32()
+        self.L.gw_batch_get_persistent(self.h, C.byref(v))
+        return bool(v.value)
+
     def launch(self) -> tuple[int, int, int]:
         """(lanes per CRU, threads per block, CRUs per lane) of the current SSA launch shape."""
         lanes, blk, cpl = i32(), i32(), i32()
@@ -413,10 +438,8 @@ def _crus_struct(crus: dict | None):
 
 
 def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, istim, record_every: int,
-        globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0, precision: int = PREC_FP64,
-        noalloc: bool = False) -> dict:
-    """gw_run: the run<PRNG> loop (src/pyGW.hpp:102-122) with ceil(num_steps / record_every) records.
-    noalloc=True calls gw_run_noalloc instead: the same run with no device allocation."""
+        globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0, precision: int = PREC_FP64) -> dict:
+    """gw_run: the run<PRNG> loop (src/pyGW.hpp:102-122) with ceil(num_steps / record_every) records."""
     R = -(-int(num_steps) // int(record_every))
     n = int(n_cru)
     out = {"t": np.zeros(R), "V": np.zeros(R), "m": np.zeros(R), "h": np.zeros(R), "j": np.zeros(R),
@@ -431,8 +454,7 @@ def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, ist
         setattr(rec, k, v.ctypes.data)
     ist = None if istim is None else np.ascontiguousarray(istim, np.float64)
     cs, keep = _crus_struct(crus)
-    fn = lib().gw_run_noalloc if noalloc else lib().gw_run
-    _call(fn, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
+    _call(lib().gw_run, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
           C.byref(globals_) if globals_ is not None else None, C.byref(cs) if cs is not None else None,
           int(seed) & (2**64 - 1), prng, precision, C.byref(rec))
     del keep

@@ -359,11 +359,13 @@ __device__ __forceinline__ void load_lcc_table(const KT &K, TT &T)
 // FP32-local variant); CaSS and CaJSR are FP64 in both.
 template <class R> struct LocalState;
 template <> struct LocalState<double> {
+    using Consts = StepConsts;
     static __device__ __forceinline__ double *jlcc(const KArgs &a) { return a.JLCC; }
     static __device__ __forceinline__ const SsaParams &params(const DevCell &C) { return C.sp; }
     static __device__ __forceinline__ const StepConsts &consts(const CellDyn &D, int ks) { return D.k[ks]; }
 };
 template <> struct LocalState<float> {
+    using Consts = StepConstsF;
     static __device__ __forceinline__ float *jlcc(const KArgs &a) { return a.JLCCf; }
     static __device__ __forceinline__ const SsaParamsF &params(const DevCell &C) { return C.spf; }
     static __device__ __forceinline__ const StepConstsF &consts(const CellDyn &D, int ks) { return D.kf[ks]; }
@@ -452,7 +454,7 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
         p.open = static_cast<int64_t>(acc.ints[1]);
         p.clch = static_cast<int64_t>(acc.ints[2]);
         p.err = acc.ints[3] ? 1 : 0;
-        p._pad = 0;
+        p.tag = 0;
         a.partials[ks * a.nblocks + b] = p;
     }
 }
@@ -481,17 +483,32 @@ static_assert(sizeof(gw_globals_t) == 36 * sizeof(double), "globals are 36 doubl
 // lane, and the 5 exponentials of the next step's constants likewise; the
 // rest is the serial arithmetic of prepare_step and lookahead_consts. Every
 // thread of the block takes part (the block synchronises).
-template <int NS>
-__device__ __forceinline__ void close_cell(const KArgs &a, int cell, CloseShared &sh)
+// D is the cell's dynamic state: a.dyn[cell], or in the persistent kernel a
+// copy the close block keeps in LDS for the whole launch. With COH the block
+// partials are read with device-coherent atomic loads (the persistent
+// kernel's SSA blocks write them that way within the launch).
+template <int NS, bool COH = false>
+__device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D, CloseShared &sh)
 {
     const DevCell &C = a.cells[cell];
-    CellDyn &D = a.dyn[cell];
     const int ks = static_cast<int>(D.step & 1);
     const int tid = threadIdx.x;
     Partial mine;
     partial_zero(mine);
     if (tid < NS)
-        for (int q = tid; q < C.nblk[a.shape]; q += NS) partial_add(mine, a.partials[ks * a.nblocks + C.blk0[a.shape] + q]);
+        for (int q = tid; q < C.nblk[a.shape]; q += NS) {
+            const Partial *src = a.partials + ks * a.nblocks + C.blk0[a.shape] + q;
+            if constexpr (COH) {
+                Partial p;
+                unsigned long long *pw = reinterpret_cast<unsigned long long *>(&p);
+                const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(src);
+                for (int t = 0; t < static_cast<int>(sizeof(Partial) / 8); ++t)
+                    pw[t] = __hip_atomic_load(sw + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                partial_add(mine, p);
+            } else {
+                partial_add(mine, *src);
+            }
+        }
     reduce_slots<NS>(sh.red, mine, tid, tid < NS);
     const Partial tot = sh.red[0];
     const int status = D.status;
@@ -610,7 +627,7 @@ __global__ __launch_bounds__(BLOCK, kWavesPerSimd<R>) void k_step(KArgs a, int64
     __shared__ BlockAcc acc;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kWave>(a, bid, csh);
+        close_cell<kWave>(a, bid, a.dyn[bid], csh);
         return;
     }
     ssa_block<PRNG, DIGEST, R, BLOCK>(a, bid - nclose, base ? *base + ssa_step : ssa_step, 1, T, acc);
@@ -714,7 +731,7 @@ __device__ __forceinline__ void ssa_block_refill(const KArgs &a, int b, int64_t 
         p.open = static_cast<int64_t>(acc.ints[1]);
         p.clch = static_cast<int64_t>(acc.ints[2]);
         p.err = acc.ints[3] ? 1 : 0;
-        p._pad = 0;
+        p.tag = 0;
         a.partials[ks * a.nblocks + b] = p;
     }
 }
@@ -731,7 +748,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_step_refill(KArgs a, int64_t ssa_s
     __shared__ BlockAcc acc;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kWave>(a, bid, csh);
+        close_cell<kWave>(a, bid, a.dyn[bid], csh);
         return;
     }
     ssa_block_refill<PRNG, R, BLOCK, CPT>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc);
@@ -920,7 +937,7 @@ __device__ __forceinline__ void ssa_block_lane(const KArgs &a, int b, int64_t st
         p.open = static_cast<int64_t>(acc.ints[1]);
         p.clch = static_cast<int64_t>(acc.ints[2]);
         p.err = acc.ints[3] ? 1 : 0;
-        p._pad = 0;
+        p.tag = 0;
         a.partials[ks * a.nblocks + b] = p;
     }
 }
@@ -936,10 +953,285 @@ __global__ __launch_bounds__(BLOCK, 4) void k_step_lane(KArgs a, int64_t ssa_ste
     __shared__ LaneSums<BLOCK> Ls;
     const int bid = blockIdx.x;
     if (bid < nclose) {
-        close_cell<kWave>(a, bid, csh);
+        close_cell<kWave>(a, bid, a.dyn[bid], csh);
         return;
     }
     ssa_block_lane<PRNG, BLOCK, CPT>(a, bid - nclose, base ? *base + ssa_step : ssa_step, T, acc, Ls);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent stepping of a batch whose SSA blocks can all be resident at once
+// (a few cells of up to tens of thousands of CRUs, where a step is short and a
+// launch per step leaves the chip waiting on launch latency). One launch runs
+// nsteps Euler steps with the cells' CRUs held in registers. The order of the
+// work is that of the launch-per-step form: SSA(n) of a cell runs with the
+// constants formed by the close of step n - 2, beside the close of step n - 1,
+// and the close of step n sums the block partials of SSA(n). Within the
+// launch the kernel boundaries are replaced by tags and a counter: each SSA
+// block writes its partial and then the partial's tag (the step index + 1);
+// the cell's close block waits until every partial of the cell carries the
+// tag of the step it closes, and after forming the constants of step n + 2
+// it publishes them and raises the cell's `kready` to n + 3, which the SSA
+// blocks wait for before step n + 2. (A counter that every SSA block
+// incremented would serialise hundreds of same-address atomics per step.)
+
+// Everything that crosses blocks (partials, constants, counters) is written
+// and read with device-coherent atomic accesses, and each writer waits for its
+// data stores to complete before it moves a counter, so no cache-wide
+// writeback or invalidation is issued. The last close runs in the same launch,
+// so every step is closed when it returns. The results are bit-identical to
+// the launch-per-step form (the same arithmetic, exact flux sums). Every wait
+// is bounded: after a.spin_limit polls the waiter sets `timeout`, which every
+// other waiter also sees, and all blocks leave.
+// ---------------------------------------------------------------------------
+constexpr int64_t kSpinLimit = int64_t(1) << 21;   // polls of about 1 us: a wait gives up after seconds
+
+__device__ __forceinline__ unsigned long long ld_coh(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until this thread's stores have completed (the counters move after).
+__device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ bool wait_at_least(unsigned long long *p, unsigned long long v, int32_t *timeout, int64_t limit)
+{
+    for (int64_t it = 0;; ++it) {
+        if (ld_coh(p) >= v) return true;
+        if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (it >= limit) {
+            __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Counter, published constants and partial tags of every cell before a
+// persistent launch from the fully closed state at step step0, whose
+// constants of steps step0 and step0 + 1 are in the cell's parity slots. One
+// block per cell.
+template <class R>
+__global__ void k_sync_init(KArgs a, int64_t step0)
+{
+    using KT = typename LocalState<R>::Consts;
+    constexpr int kWords = sizeof(KT) / 8;
+    const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x == 0) *a.timeout = 0;
+    if (c >= a.ncells) return;
+    CellSync &X = a.sync[c];
+    const CellDyn &D = a.dyn[c];
+    for (int t = threadIdx.x; t < 2 * kXchgWords; t += blockDim.x) {
+        const int par = t / kXchgWords, w = t % kXchgWords;
+        unsigned long long v = 0;
+        if (w < kWords) v = reinterpret_cast<const unsigned long long *>(&LocalState<R>::consts(D, par))[w];
+        else if (w == kXchgWords - 1) v = static_cast<unsigned long long>(D.stop_step);
+        X.k[par][w] = v;
+    }
+    if (threadIdx.x == 0) X.kready = static_cast<unsigned long long>(step0 + 2);
+    const DevCell &C = a.cells[c];
+    for (int q = threadIdx.x; q < 2 * C.nblk[kShapeDefault]; q += blockDim.x) {
+        const int par = q & 1, blk = q >> 1;
+        a.partials[par * a.nblocks + C.blk0[kShapeDefault] + blk].tag = 0;
+    }
+}
+
+template <int PRNG, bool DIGEST, class R>
+__device__ __forceinline__ void ssa_block_persist(const KArgs &a, int b, int64_t step0, int64_t nsteps,
+                                                  LccTableT<R> &T, BlockAcc &acc,
+                                                  typename LocalState<R>::Consts &K, int64_t &stop, int &leave)
+{
+    using LS = LocalState<R>;
+    using KT = typename LS::Consts;
+    constexpr int kWords = sizeof(KT) / 8;
+    static_assert(sizeof(KT) % 8 == 0 && kWords < kXchgWords, "step constants fit the exchange slot");
+    const int tid = threadIdx.x;
+    const int j = tid & 3;
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const DevCell &C = a.cells[cell];
+    CellSync &X = a.sync[cell];
+    const auto &sp = LS::params(C);
+    const int64_t i = (b - C.blk0[kShapeDefault]) * (kBlock / 4) + (tid >> 2);
+    const bool mine = i < C.n_cru;
+    const int64_t slot = C.slot0 + (mine ? i : 0);
+    R *const JLCC = LS::jlcc(a);
+    double ca = 0.0;
+    R jl = R(0);
+    uint64_t w = 0;
+    CruQ q{};
+    Digest dg{0, 0};
+    if (mine) {
+        ca = a.CaSS[slot * 4 + j];
+        jl = JLCC[slot * 4 + j];
+        w = a.sub[slot * 4 + j];
+        q.jsr = a.CaJSR[slot];
+        q.r0 = a.rng[slot * 4];
+        q.r1 = a.rng[slot * 4 + 1];
+        q.r2 = a.rng[slot * 4 + 2];
+        q.r3 = a.rng[slot * 4 + 3];
+        if (DIGEST) {
+            dg.count = a.dig[slot * 2];
+            dg.hash = a.dig[slot * 2 + 1];
+        }
+    }
+    int changed = 0;
+    const R dt = static_cast<R>(a.dt);
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t n = step0 + k;
+        const int ks = static_cast<int>(n & 1);
+        if (tid == 0)
+            leave = wait_at_least(&X.kready, static_cast<unsigned long long>(n + 1), a.timeout, a.spin_limit) ? 0 : 1;
+        __syncthreads();
+        if (leave) break;
+        // the step's constants and the cell's stop step, published by its close
+        for (int t = tid; t < kWords; t += kBlock) reinterpret_cast<unsigned long long *>(&K)[t] = ld_coh(&X.k[ks][t]);
+        if (tid == 0) stop = static_cast<int64_t>(ld_coh(&X.k[ks][kXchgWords - 1]));
+        acc_zero(acc);
+        __syncthreads();
+        load_lcc_table(K, T);
+        __syncthreads();
+        if (mine && n < stop) {
+            const QuadOut o = ssa_step_quad<PRNG, DIGEST>(ca, jl, w, q, sp, K, T, dt, dg, static_cast<uint32_t>(n & 0xffff), j);
+            changed |= o.changed;
+            // lane j adds output j of its CRU (update_CRUstate_from_temp, GW.tpp:47-62)
+            const double v = j == 0 ? o.Jxfer : j == 1 ? o.Jtr : j == 2 ? o.JLCC : o.Jrel;
+            Fx f;
+            fx_zero(f);
+            const int fl = fx_add(f, v);
+            acc_add_fx(acc, j, f);
+            const int e = (qgeti<0>(fl) | qgeti<1>(fl) | qgeti<2>(fl)) | (o.err ? 4 : 0);
+            const int64_t iv = j == 0 ? static_cast<int64_t>(o.iters) : j == 1 ? o.open : j == 2 ? o.clch : e;
+            atomicAdd(&acc.ints[j], static_cast<unsigned long long>(iv));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            Partial p;
+            p.jxfer = acc_fx(acc, 0);
+            p.jtr = acc_fx(acc, 1);
+            p.jlcc = acc_fx(acc, 2);
+            p.jrel = acc_fx(acc, 3);
+            p.iters = acc.ints[0];
+            p.open = static_cast<int64_t>(acc.ints[1]);
+            p.clch = static_cast<int64_t>(acc.ints[2]);
+            p.err = acc.ints[3] ? 1 : 0;
+            p.tag = static_cast<int32_t>(n + 1);
+            const unsigned long long *pw = reinterpret_cast<const unsigned long long *>(&p);
+            unsigned long long *dst = reinterpret_cast<unsigned long long *>(a.partials + ks * a.nblocks + b);
+            constexpr int kLast = static_cast<int>(sizeof(Partial) / 8) - 1;   // the word holding err and tag
+            for (int t = 0; t < kLast; ++t) st_coh(dst + t, pw[t]);
+            stores_done();
+            st_coh(dst + kLast, pw[kLast]);
+        }
+    }
+    if (mine) {
+        a.CaSS[slot * 4 + j] = ca;
+        if (changed) {
+            JLCC[slot * 4 + j] = jl;
+            a.sub[slot * 4 + j] = w;
+        }
+        if (j == 0) {
+            a.CaJSR[slot] = q.jsr;
+            a.rng[slot * 4] = q.r0;
+            a.rng[slot * 4 + 1] = q.r1;
+            a.rng[slot * 4 + 2] = q.r2;
+            a.rng[slot * 4 + 3] = q.r3;
+            if (DIGEST) {
+                a.dig[slot * 2] = dg.count;
+                a.dig[slot * 2 + 1] = dg.hash;
+            }
+        }
+    }
+}
+
+// Wave 0 of a close block waits until every partial of the cell for step n
+// carries the tag n + 1 (each lane polls its share, the wave votes). The
+// partials of a parity are cleared before the launch (k_sync_init), and
+// SSA(n + 2), which writes the same slots, waits for the close of step n.
+__device__ bool wait_partials(const KArgs &a, const DevCell &C, int64_t n, int32_t *timeout, int64_t limit)
+{
+    constexpr int kTagWord = static_cast<int>(sizeof(Partial) / 8) - 1;
+    const int nblk = C.nblk[kShapeDefault];
+    const unsigned long long *base =
+        reinterpret_cast<const unsigned long long *>(a.partials + (n & 1) * a.nblocks + C.blk0[kShapeDefault]);
+    const int32_t want = static_cast<int32_t>(n + 1);
+    for (int64_t it = 0;; ++it) {
+        bool ok = true;
+        for (int q = threadIdx.x; q < nblk; q += kWave)
+            ok = ok && static_cast<int32_t>(ld_coh(base + q * (sizeof(Partial) / 8) + kTagWord) >> 32) == want;
+        if (__all(ok)) return true;
+        if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (it >= limit) {
+            if (threadIdx.x == 0) __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// The close block of one cell in the persistent kernel: keeps the cell's
+// dynamic state in LDS for the launch (it is its only writer), closes steps
+// step0 .. step0 + nsteps - 1 in turn, each once every SSA block of the cell
+// has written its partial of that step, and publishes the constants it
+// formed.
+template <class R>
+__device__ __forceinline__ void close_persist(const KArgs &a, int cell, int64_t step0, int64_t nsteps, CellDyn &Dl,
+                                              CloseShared &sh, int &leave)
+{
+    using KT = typename LocalState<R>::Consts;
+    constexpr int kWords = sizeof(KT) / 8;
+    static_assert(sizeof(CellDyn) % 8 == 0, "the dynamic state is whole 8-byte words");
+    constexpr int kDynWords = sizeof(CellDyn) / 8;
+    const DevCell &C = a.cells[cell];
+    CellSync &X = a.sync[cell];
+    unsigned long long *g = reinterpret_cast<unsigned long long *>(&a.dyn[cell]);
+    unsigned long long *l = reinterpret_cast<unsigned long long *>(&Dl);
+    for (int t = threadIdx.x; t < kDynWords; t += blockDim.x) l[t] = g[t];
+    __syncthreads();
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t n = step0 + k;
+        if (threadIdx.x < kWave) {
+            const bool ok = wait_partials(a, C, n, a.timeout, a.spin_limit);
+            if (threadIdx.x == 0) leave = ok ? 0 : 1;
+        }
+        __syncthreads();
+        if (leave) break;
+        close_cell<kWave, true>(a, cell, Dl, sh);
+        __syncthreads();
+        // publish the constants of step n + 2 and the stop step, then the counter
+        const int par = static_cast<int>((n + 2) & 1);
+        for (int t = threadIdx.x; t < kWords; t += blockDim.x)
+            st_coh(&X.k[par][t], reinterpret_cast<const unsigned long long *>(&LocalState<R>::consts(Dl, par))[t]);
+        if (threadIdx.x == 0) st_coh(&X.k[par][kXchgWords - 1], static_cast<unsigned long long>(Dl.stop_step));
+        stores_done();
+        __syncthreads();
+        if (threadIdx.x == 0) st_coh(&X.kready, static_cast<unsigned long long>(n + 3));
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kDynWords; t += blockDim.x) g[t] = l[t];
+}
+
+// The first ncells blocks close (their first wave does the arithmetic), the
+// others are the SSA blocks of the default shape (64 CRUs, a quad of lanes
+// per CRU).
+template <int PRNG, bool DIGEST, class R>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_persist(KArgs a, int64_t step0, int64_t nsteps)
+{
+    __shared__ LccTableT<R> T;
+    __shared__ CloseShared csh;
+    __shared__ BlockAcc acc;
+    __shared__ typename LocalState<R>::Consts K;
+    __shared__ CellDyn Dl;
+    __shared__ int64_t stop;
+    __shared__ int leave;
+    const int bid = blockIdx.x;
+    if (bid < a.ncells) {
+        close_persist<R>(a, bid, step0, nsteps, Dl, csh, leave);
+        return;
+    }
+    ssa_block_persist<PRNG, DIGEST, R>(a, bid - a.ncells, step0, nsteps, T, acc, K, stop, leave);
 }
 
 // Graph bookkeeping: the step index a replayed chunk starts at.
@@ -1062,11 +1354,61 @@ __global__ void k_reset_obs(CellDyn *dyn, int ncells)
     o.apd90_last = -1.0; o.apd90_prev = -1.0; o.alt_sum = 0; o.alt_count = 0; o.beats_done = 0;
 }
 
-// Status and trace row count of every cell into out[0..n) and out[n..2n), so
-// that the host reads them with one copy.
-__global__ void k_gather_dyn(const CellDyn *dyn, int ncells, int64_t *out)
+// The post-hoc L-type Ca and Ca-activated Cl currents of src/model.py:414-481
+// at each cell's current state (one block per cell): ICaL from the open LCCs
+// (state 6 or 12 with the y gate open) at the instantaneous V and CaSS,
+//   ICaL = CRU_factor PCaL/CSA 4F (VF/RT)/(e^{2VF/RT} - 1) sum_open (CaSS e^{2VF/RT} - 0.341 Cao) 1e9,
+// and Ito2 from the open ClCh,
+//   Ito2 = CRU_factor Pto2/CSA F (VF/RT)/(e^{-VF/RT} - 1) n_open (Clcyto e^{-VF/RT} - Clo) 1e9.
+// The sum of CaSS over open channels is exact (fixed point), so the result
+// does not depend on the launch. At V = 0 the factors take their limits,
+// 1/2 and -1, where the reference divides 0/0 (deviation D4). out[2c] = ICaL,
+// out[2c + 1] = Ito2 of cell c.
+__global__ void k_posthoc_currents(KArgs a, double *out)
+{
+    __shared__ Partial red[kBlock];
+    const int cell = blockIdx.x;
+    const DevCell &C = a.cells[cell];
+    const CellDyn &D = a.dyn[cell];
+    Partial mine;
+    partial_zero(mine);
+    for (int64_t i = threadIdx.x; i < C.n_cru; i += kBlock) {
+        const int64_t slot = C.slot0 + i;
+        double s = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t w = a.sub[slot * 4 + j];
+            const int L = sub_lcc(w);
+            if ((L == 6 || L == 12) && sub_y(w)) {
+                s += ld_cass(a, slot * 4 + j);
+                mine.open += 1;
+            }
+            mine.clch += sub_cl(w);
+        }
+        mine.err |= fx_add(mine.jlcc, s);
+    }
+    block_reduce<kBlock>(red, mine);
+    if (threadIdx.x != 0) return;
+    const Partial &t = red[0];
+    const gw_params_t &p = C.p;
+    const double V = D.g.V;
+    const double x = V * C.c.F_RT;
+    const double e2 = gw::exp(2.0 * x), em = gw::exp(-x);
+    const double f2 = (V == 0.0) ? 0.5 : x / (e2 - 1.0);
+    const double fm = (V == 0.0) ? -1.0 : x / (em - 1.0);
+    const double n_open = static_cast<double>(t.open);
+    out[2 * cell] = C.c.CRU_factor * (p.PCaL / p.CSA) * (4.0 * kFaraday) * f2 *
+                    (fx_to_double(t.jlcc) * e2 - 0.341 * p.Cao * n_open) * 1e9;
+    out[2 * cell + 1] = C.c.CRU_factor * (p.Pto2 / p.CSA) * kFaraday * fm *
+                        static_cast<double>(t.clch) * (p.Clcyto * em - p.Clo) * 1e9;
+}
+
+// Status and trace row count of every cell into out[0..n) and out[n..2n),
+// and the persistent kernel's timeout flag into out[2n], so that the host
+// reads them with one copy.
+__global__ void k_gather_dyn(const CellDyn *dyn, int ncells, const int32_t *timeout, int64_t *out)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) out[2 * ncells] = *timeout;
     if (c >= ncells) return;
     out[c] = dyn[c].status;
     out[ncells + c] = dyn[c].trace_rows;
@@ -1186,6 +1528,8 @@ struct gw_batch {
     int32_t trace_every = 0;
     int64_t trace_cap = 0;
     int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
+    int32_t persist_mode = 0;        // persistent step kernel: 0 off (default), 1 on, 2 when every block fits at once
+    int32_t persist_fit = -1;        // whether every block of the persistent kernel fits at once (-1: not known)
     int32_t cfg = 2;                 // launch configuration (kConfigs), 4 x 256 by default
     // replayed graphs of step chunks (0 = plain launches)
     int32_t graph_steps = 0;
@@ -1209,18 +1553,57 @@ struct gw_batch {
     void *d_staging = nullptr;
     size_t staging_bytes = 0;
     int64_t *d_gather = nullptr;   // [2][ncells]: status, trace rows (k_gather_dyn)
+    // allocation-free batches carve their device memory from the scratch array
+    // instead of allocating it, and free nothing
+    char *arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
+    double *arena_table = nullptr;   // the stimulus table of cell 0
+    int64_t arena_table_cap = 0;
 };
 
 namespace {
 
+// Device memory of a batch: hipMalloc, or for an allocation-free batch the
+// next 256-byte aligned piece of its arena.
+int carve(gw_batch *B, void **p, size_t bytes, char *err, size_t errlen)
+{
+    const size_t at = (B->arena_used + 255) & ~size_t(255);
+    if (at + bytes > B->arena_size) {
+        set_err(err, errlen, "the allocation-free path holds at most %zu bytes of device state (%zu MiB of static "
+                "scratch per GPU); this run needs more: use fewer CRUs or fewer steps, or gw_run",
+                B->arena_size, B->arena_size >> 20);
+        return GW_E_INVALID_ARG;
+    }
+    *p = B->arena + at;
+    B->arena_used = at + bytes;
+    return GW_OK;
+}
+
 template <class T>
-int dmalloc(T **p, size_t n, char *err, size_t errlen)
+int dmalloc(gw_batch *B, T **p, size_t n, char *err, size_t errlen)
 {
     *p = nullptr;
     if (n == 0) return GW_OK;
+    if (B->arena) return carve(B, reinterpret_cast<void **>(p), n * sizeof(T), err, errlen);
     HIP_TRY(hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)));
     return GW_OK;
 }
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_i32(int32_t *p, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, int32_t prng_id,
+                 double dt, const gw_globals_t *g0, void *arena, size_t arena_size, int64_t table_cap,
+                 gw_batch **out, char *err, size_t errlen);
 
 int validate_params(const gw_params_t &p, char *err, size_t errlen)
 {
@@ -1461,6 +1844,64 @@ hipError_t graph_build(gw_batch *B)
     return hipSuccess;
 }
 
+// The persistent step kernel of a batch's engine and precision.
+template <int PRNG>
+const void *persist_fn_prng(const gw_batch *B)
+{
+    if (B->precision == GW_PREC_FP32_LOCAL) return reinterpret_cast<const void *>(&k_persist<PRNG, false, float>);
+    if (B->digest) return reinterpret_cast<const void *>(&k_persist<PRNG, true, double>);
+    return reinterpret_cast<const void *>(&k_persist<PRNG, false, double>);
+}
+
+const void *persist_fn(const gw_batch *B)
+{
+    switch (B->prng) {
+    case 0: return persist_fn_prng<0>(B);
+    case 1: return persist_fn_prng<1>(B);
+    case 2: return persist_fn_prng<2>(B);
+    case 3: return persist_fn_prng<3>(B);
+    case 4: return persist_fn_prng<4>(B);
+    default: return persist_fn_prng<5>(B);
+    }
+}
+
+// Whether every block of the persistent kernel (the closes and the SSA blocks
+// of the default shape) can be resident at once on the batch's device.
+bool persist_fits(gw_batch *B)
+{
+    if (B->persist_fit >= 0) return B->persist_fit != 0;
+    int per_cu = 0, cus = 0;
+    const int64_t grid = B->ncells + B->nblocks_shape[kShapeDefault];
+    B->persist_fit = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_fn(B), kBlock, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->ctx->device) == hipSuccess)
+        B->persist_fit = static_cast<int64_t>(per_cu) * cus >= grid ? 1 : 0;
+    return B->persist_fit != 0;
+}
+
+// The persistent kernel runs the default shape's free-running steps.
+bool use_persist(gw_batch *B)
+{
+    return B->persist_mode != 0 && !B->frozen && B->cfg == kConfigDefault && persist_fits(B);
+}
+
+// Steps step0 .. step0+n-1 in one cooperative launch of the persistent kernel
+// (which closes every step itself), after its counters are set for step0.
+hipError_t launch_persist(gw_batch *B, int64_t step0, int64_t n)
+{
+    if (B->precision == GW_PREC_FP32_LOCAL)
+        hipLaunchKernelGGL(k_sync_init<float>, dim3(B->ncells), dim3(64), 0, B->ctx->stream, B->a, step0);
+    else
+        hipLaunchKernelGGL(k_sync_init<double>, dim3(B->ncells), dim3(64), 0, B->ctx->stream, B->a, step0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t grid = B->ncells + B->nblocks_shape[kShapeDefault];
+    void *args[] = {&B->a, &step0, &n};
+    e = hipLaunchCooperativeKernel(persist_fn(B), dim3(static_cast<unsigned>(grid)), dim3(kBlock), args, 0,
+                                   B->ctx->stream);
+    return e;
+}
+
 // Steps step0 .. step0+n-1: n launches, launch k running SSA(step0+k) beside
 // close(step0+k-1), then one launch with the last closes. Between calls every
 // step is closed, so the cell state read by the host is that of step0+n.
@@ -1468,6 +1909,7 @@ hipError_t graph_build(gw_batch *B)
 hipError_t launch_steps(gw_batch *B, int64_t step0, int64_t n)
 {
     if (n <= 0) return hipSuccess;
+    if (use_persist(B)) return launch_persist(B, step0, n);
     launch_one(B, step0, 0, nullptr, nullptr);
     int64_t k = 1;
     if (B->graph_steps > 0 && n - 1 >= B->graph_steps) {
@@ -1530,9 +1972,9 @@ int launch_steps_timed(gw_batch *B, int64_t step0, int64_t n, double *ssa_ms, ch
 // Status and trace row count of every cell, read with one copy.
 int gather_dyn(gw_batch *B, std::vector<int64_t> &out, char *err, size_t errlen)
 {
-    out.assign(2 * static_cast<size_t>(B->ncells), 0);
+    out.assign(2 * static_cast<size_t>(B->ncells) + 1, 0);
     hipLaunchKernelGGL(k_gather_dyn, dim3((B->ncells + 255) / 256), dim3(256), 0, B->ctx->stream, B->d_dyn, B->ncells,
-                       B->d_gather);
+                       B->a.timeout, B->d_gather);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out.data(), B->d_gather, out.size() * sizeof(int64_t), hipMemcpyDeviceToHost, B->ctx->stream));
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));
@@ -1553,6 +1995,10 @@ int check_status(gw_batch *B, char *err, size_t errlen)
     std::vector<int64_t> v;
     const int rc = gather_dyn(B, v, err, errlen);
     if (rc) return rc;
+    if (v[2 * static_cast<size_t>(B->ncells)] != 0) {
+        set_err(err, errlen, "a bounded wait of the persistent step kernel expired; the steps of this call are incomplete");
+        return GW_E_TIMEOUT;
+    }
     for (int c = 0; c < B->ncells; ++c) {
         const int32_t st = static_cast<int32_t>(v[c]);
         if (st != 0) {
@@ -1599,6 +2045,10 @@ int sample_rest(gw_batch *B, char *err, size_t errlen)
 int ensure_staging(gw_batch *B, size_t bytes, char *err, size_t errlen)
 {
     if (B->staging_bytes >= bytes) return GW_OK;
+    if (B->arena) {
+        set_err(err, errlen, "allocation-free batch: staging of %zu bytes exceeds the %zu carved", bytes, B->staging_bytes);
+        return GW_E_INVALID_ARG;
+    }
     if (B->d_staging) HIP_TRY(hipFree(B->d_staging));
     B->d_staging = nullptr;
     B->staging_bytes = 0;
@@ -1741,9 +2191,13 @@ int gw_batch_destroy(gw_batch *B)
     (void)hipSetDevice(B->ctx->device);
     (void)hipStreamSynchronize(B->ctx->stream);
     graph_release(B);
+    if (B->arena) {   // carved from the scratch array: nothing to free
+        delete B;
+        return GW_OK;
+    }
     void *ptrs[] = {B->a.CaSS, B->a.CaJSR, B->a.JLCC, B->a.JLCCf, B->a.sub, B->a.rng, B->a.dig, B->a.partials,
                     B->a.trace, B->a.iters_total, B->d_step_base, B->d_cells, B->d_dyn, B->d_block_cell, B->d_staging,
-                    B->d_gather};
+                    B->d_gather, B->a.sync, B->a.timeout};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : B->istim_dev)
@@ -1754,6 +2208,20 @@ int gw_batch_destroy(gw_batch *B)
 
 int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, int32_t prng_id,
                     double dt, const gw_globals_t *g0, gw_batch **out, char *err, size_t errlen)
+{
+    return batch_create(ctx, ncells, cells, precision, prng_id, dt, g0, nullptr, 0, 0, out, err, errlen);
+}
+
+}  // extern "C"
+
+namespace {
+
+// gw_batch_create, or with arena != null an allocation-free batch whose device
+// memory is carved from arena[0, arena_size), with a stimulus table of
+// table_cap steps and the staging of one cell's state carved beside it.
+int batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, int32_t precision, int32_t prng_id,
+                 double dt, const gw_globals_t *g0, void *arena, size_t arena_size, int64_t table_cap,
+                 gw_batch **out, char *err, size_t errlen)
 {
     if (!ctx || !cells || !out || ncells <= 0) {
         set_err(err, errlen, "gw_batch_create: null argument or ncells <= 0");
@@ -1775,6 +2243,8 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     HIP_TRY(hipSetDevice(ctx->device));
     gw_batch *B = new gw_batch;
     B->ctx = ctx;
+    B->arena = static_cast<char *>(arena);
+    B->arena_size = arena ? arena_size : 0;
     B->ncells = ncells;
     B->precision = precision;
     B->prng = prng_id;
@@ -1833,18 +2303,34 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     KArgs &a = B->a;
     int rc = GW_OK;
     const bool f32 = precision == GW_PREC_FP32_LOCAL;
-    if ((rc = dmalloc(&a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(&a.CaJSR, N, err, errlen)) ||
-        (!f32 && (rc = dmalloc(&a.JLCC, 4 * N, err, errlen))) ||
-        (f32 && (rc = dmalloc(&a.JLCCf, 4 * N, err, errlen))) ||
-        (rc = dmalloc(&a.sub, 4 * N, err, errlen)) ||
-        (rc = dmalloc(&a.rng, 4 * N, err, errlen)) ||
-        (!f32 && (rc = dmalloc(&a.dig, 2 * N, err, errlen))) || (rc = dmalloc(&a.partials, 2 * B->max_nblocks, err, errlen)) ||
-        (rc = dmalloc(&a.iters_total, 1, err, errlen)) || (rc = dmalloc(&B->d_step_base, 1, err, errlen)) ||
-        (rc = dmalloc(&B->d_cells, ncells, err, errlen)) || (rc = dmalloc(&B->d_dyn, ncells, err, errlen)) ||
-        (rc = dmalloc(&B->d_gather, 2 * static_cast<size_t>(ncells), err, errlen)) ||
-        (rc = dmalloc(&B->d_block_cell, B->block_cell_h.size(), err, errlen))) {
+    if ((rc = dmalloc(B, &a.CaSS, 4 * N, err, errlen)) || (rc = dmalloc(B, &a.CaJSR, N, err, errlen)) ||
+        (!f32 && (rc = dmalloc(B, &a.JLCC, 4 * N, err, errlen))) ||
+        (f32 && (rc = dmalloc(B, &a.JLCCf, 4 * N, err, errlen))) ||
+        (rc = dmalloc(B, &a.sub, 4 * N, err, errlen)) ||
+        (rc = dmalloc(B, &a.rng, 4 * N, err, errlen)) ||
+        (!f32 && (rc = dmalloc(B, &a.dig, 2 * N, err, errlen))) ||
+        (rc = dmalloc(B, &a.partials, 2 * B->max_nblocks, err, errlen)) ||
+        (rc = dmalloc(B, &a.iters_total, 1, err, errlen)) || (rc = dmalloc(B, &B->d_step_base, 1, err, errlen)) ||
+        (rc = dmalloc(B, &B->d_cells, ncells, err, errlen)) || (rc = dmalloc(B, &B->d_dyn, ncells, err, errlen)) ||
+        (rc = dmalloc(B, &B->d_gather, 2 * static_cast<size_t>(ncells) + 1, err, errlen)) ||
+        (rc = dmalloc(B, &a.sync, static_cast<size_t>(ncells), err, errlen)) ||
+        (rc = dmalloc(B, &a.timeout, 1, err, errlen)) ||
+        (rc = dmalloc(B, &B->d_block_cell, B->block_cell_h.size(), err, errlen))) {
         gw_batch_destroy(B);
         return rc;
+    }
+    if (B->arena) {
+        // the staging of one cell's state (set/get_cru_state) and the stimulus table
+        void *stg = nullptr;
+        const size_t sb = staging_size(B->cells_h[0].n_cru);
+        if ((rc = carve(B, &stg, sb, err, errlen)) ||
+            (table_cap > 0 && (rc = dmalloc(B, &B->arena_table, static_cast<size_t>(table_cap), err, errlen)))) {
+            gw_batch_destroy(B);
+            return rc;
+        }
+        B->d_staging = stg;
+        B->staging_bytes = sb;
+        B->arena_table_cap = table_cap;
     }
     a.cells = B->d_cells;
     a.dyn = B->d_dyn;
@@ -1860,6 +2346,7 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     a.trace = nullptr;
     a.prng = prng_id;
     a.precision = precision;
+    a.spin_limit = kSpinLimit;
     // per-cell stimulus tables are attached later; globals start at g0 or the defaults
     std::vector<CellDyn> dyn(ncells);
     B->g0.resize(ncells);
@@ -1878,7 +2365,14 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     if ((e = hipMemcpyAsync(B->d_cells, B->cells_h.data(), ncells * sizeof(DevCell), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(B->d_dyn, dyn.data(), ncells * sizeof(CellDyn), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(B->d_block_cell, B->block_cell_h.data(), B->block_cell_h.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return fail(e);
-    if ((e = hipMemsetAsync(a.iters_total, 0, sizeof(unsigned long long), ctx->stream)) != hipSuccess) return fail(e);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, a.iters_total, int64_t(1), 0ull);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    hipLaunchKernelGGL(k_fill_u64, dim3((ncells * (sizeof(CellSync) / 8) + 255) / 256), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<unsigned long long *>(a.sync), int64_t(ncells) * int64_t(sizeof(CellSync) / 8),
+                       0ull);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, a.timeout, int64_t(1));
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     hipLaunchKernelGGL(k_seed, dim3(B->nblocks), dim3(kBlock), 0, ctx->stream, a);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if ((rc = sample_rest(B, err, errlen)) != GW_OK) {
@@ -1890,6 +2384,10 @@ int gw_batch_create(gw_ctx *ctx, int32_t ncells, const gw_cell_desc_t *cells, in
     *out = B;
     return GW_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int gw_batch_set_cru_state(gw_batch *B, int32_t cell, const gw_cru_state_t *c, char *err, size_t errlen)
 {
@@ -2028,14 +2526,26 @@ int gw_batch_set_stimulus_table(gw_batch *B, int32_t cell, const double *istim, 
         return GW_E_INVALID_ARG;
     }
     HIP_TRY(hipSetDevice(B->ctx->device));
-    if (B->istim_dev[cell]) HIP_TRY(hipFree(B->istim_dev[cell]));
-    B->istim_dev[cell] = nullptr;
     double *d = nullptr;
-    if (n > 0) {
-        HIP_TRY(hipMalloc(&d, n * sizeof(double)));
-        HIP_TRY(hipMemcpyAsync(d, istim, n * sizeof(double), hipMemcpyHostToDevice, B->ctx->stream));
+    if (B->arena) {   // the table carved at creation (cell 0 only)
+        if (cell != 0 || n > B->arena_table_cap) {
+            set_err(err, errlen, "allocation-free batch: a stimulus table of %lld steps exceeds the %lld carved",
+                    static_cast<long long>(n), static_cast<long long>(B->arena_table_cap));
+            return GW_E_INVALID_ARG;
+        }
+        if (n > 0) {
+            d = B->arena_table;
+            HIP_TRY(hipMemcpyAsync(d, istim, n * sizeof(double), hipMemcpyHostToDevice, B->ctx->stream));
+        }
+    } else {
+        if (B->istim_dev[cell]) HIP_TRY(hipFree(B->istim_dev[cell]));
+        B->istim_dev[cell] = nullptr;
+        if (n > 0) {
+            HIP_TRY(hipMalloc(&d, n * sizeof(double)));
+            HIP_TRY(hipMemcpyAsync(d, istim, n * sizeof(double), hipMemcpyHostToDevice, B->ctx->stream));
+        }
+        B->istim_dev[cell] = d;
     }
-    B->istim_dev[cell] = d;
     B->cells_h[cell].istim = d;
     B->cells_h[cell].istim_len = n;
     B->cells_h[cell].proto.kind = GW_PROTO_TABLE;
@@ -2382,7 +2892,8 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, B->ctx->stream));
-    if (ssa_ms) {
+    const bool persist = use_persist(B);
+    if (ssa_ms && !persist) {
         rc = launch_steps_timed(B, B->steps_done, nsteps, ssa_ms, err, errlen);
         if (rc) return rc;
     } else {
@@ -2397,8 +2908,68 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = f;
+    if (ssa_ms && persist) *ssa_ms = f;   // one launch runs every step
     if (launches) *launches = nsteps;
     return check_status(B, err, errlen);
+}
+
+int gw_batch_set_persistent(gw_batch *B, int32_t mode, char *err, size_t errlen)
+{
+    if (!B || mode < 0 || mode > 2) {
+        set_err(err, errlen, "gw_batch_set_persistent: mode must be 0 (off), 1 (on) or 2 (when it fits)");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    if (mode == 1 && !persist_fits(B)) {
+        set_err(err, errlen, "gw_batch_set_persistent: the batch's %lld SSA blocks and %d close blocks cannot all be "
+                "resident at once", static_cast<long long>(B->nblocks_shape[kShapeDefault]), B->ncells);
+        return GW_E_INVALID_ARG;
+    }
+    B->persist_mode = mode;
+    return GW_OK;
+}
+
+int gw_batch_get_currents(gw_batch *B, double *ICaL, double *Ito2, char *err, size_t errlen)
+{
+    if (!B || (!ICaL && !Ito2)) {
+        set_err(err, errlen, "gw_batch_get_currents: bad arguments");
+        return GW_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    int rc = prepare(B, err, errlen);
+    if (rc) return rc;
+    const size_t bytes = 2 * sizeof(double) * static_cast<size_t>(B->ncells);
+    rc = ensure_staging(B, bytes, err, errlen);
+    if (rc) return rc;
+    double *d = static_cast<double *>(B->d_staging);
+    hipLaunchKernelGGL(k_posthoc_currents, dim3(B->ncells), dim3(kBlock), 0, B->ctx->stream, B->a, d);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(2 * static_cast<size_t>(B->ncells));
+    HIP_TRY(hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    for (int c = 0; c < B->ncells; ++c) {
+        if (ICaL) ICaL[c] = h[2 * c];
+        if (Ito2) Ito2[c] = h[2 * c + 1];
+    }
+    return GW_OK;
+}
+
+int gw_batch_set_spin_limit(gw_batch *B, int64_t polls, char *err, size_t errlen)
+{
+    if (!B || polls < 0) {
+        set_err(err, errlen, "gw_batch_set_spin_limit: polls must be >= 0");
+        return GW_E_INVALID_ARG;
+    }
+    B->a.spin_limit = polls > 0 ? polls : kSpinLimit;
+    return GW_OK;
+}
+
+int gw_batch_get_persistent(gw_batch *B, int32_t *active)
+{
+    if (!B || !active) return GW_E_INVALID_ARG;
+    (void)hipSetDevice(B->ctx->device);
+    *active = use_persist(B) ? 1 : 0;
+    return GW_OK;
 }
 
 int gw_batch_set_graph(gw_batch *B, int32_t steps_per_graph, char *err, size_t errlen)

@@ -63,7 +63,19 @@ struct Partial {
     int64_t clch, open;
     uint64_t iters;
     int32_t err;
-    int32_t _pad;
+    int32_t tag;   // the persistent kernel's step tag (step index + 1); 0 elsewhere
+};
+
+// Per-cell exchange of the persistent step kernel: a counter, then the SSA
+// constants of the latest steps the cell's close formed and the cell's stop
+// step. It is written and read only with device-coherent atomic
+// accesses, so no cache writeback or invalidation is needed between blocks
+// on different XCDs.
+constexpr int kXchgWords = 64;   // 512 bytes: StepConsts (59 words) or StepConstsF (31), then stop_step
+struct CellSync {
+    unsigned long long kready;       // the SSA constants of every step below this index are formed
+    unsigned long long _pad[7];
+    unsigned long long k[2][kXchgWords];   // constants of steps by parity; word kXchgWords - 1 holds stop_step
 };
 
 struct KArgs {
@@ -90,6 +102,10 @@ struct KArgs {
     int32_t prng;
     int32_t precision;   // GW_PREC_FP64 or GW_PREC_FP32_LOCAL
     int32_t shape;       // current SSA launch shape (0..3)
+    int32_t _pad;
+    CellSync *sync;      // [ncells] (persistent step kernel)
+    int32_t *timeout;    // set when a bounded wait of the persistent kernel expires
+    int64_t spin_limit;  // polls before a wait of the persistent kernel gives up
 };
 
 // CRU-local state of either precision, for the kernels outside the SSA.

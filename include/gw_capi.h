@@ -301,6 +301,16 @@ int gw_batch_set_cru_state(gw_batch *b, int32_t cell, const gw_cru_state_t *c,
                            char *err, size_t errlen);
 int gw_batch_get_cru_state(gw_batch *b, int32_t cell, gw_cru_state_t *c,
                            char *err, size_t errlen);
+/*
+ * The post-hoc currents of GWSolution.ICaL / .Ito2 (src/model.py:414-481) at
+ * each cell's current state, formed on the device: ICaL from the open LCCs
+ * at the instantaneous V and CaSS, Ito2 from the open ClCh, in pA/pF. (The
+ * integrator's own I_CaL, used for dV/dt and tracked in gw_cell_obs_t and the
+ * trace, freezes each channel's flux at its opening, src/includes/SSA.tpp:141;
+ * the two agree while CaSS and V stay at their values of the opening.)
+ * ICaL or Ito2 may be NULL; each has ncells entries.
+ */
+int gw_batch_get_currents(gw_batch *b, double *ICaL, double *Ito2, char *err, size_t errlen);
 /* One cell's CRU state in the reduced encoding of gw_cru_state_reduced_t. */
 int gw_batch_get_cru_state_reduced(gw_batch *b, int32_t cell, gw_cru_state_reduced_t *c,
                                    char *err, size_t errlen);
@@ -320,6 +330,31 @@ int gw_batch_set_stimulus_table(gw_batch *b, int32_t cell, const double *istim, 
  * No reference counterpart.
  */
 int gw_batch_set_graph(gw_batch *b, int32_t steps_per_graph, char *err, size_t errlen);
+/*
+ * Persistent stepping (no reference counterpart; the reference forks and joins
+ * its OpenMP loop every step, src/includes/GW.tpp:223-233). When every SSA
+ * block of the default launch shape (4 x 256) and one close block per cell can
+ * be resident on the GPU at once -- a few cells of up to tens of thousands of
+ * CRUs -- one cooperative launch runs all the steps of a gw_batch_step call
+ * with the CRUs held in registers, and the cells' SSA blocks and close blocks
+ * exchange each step's flux sums and constants through per-cell counters
+ * instead of kernel boundaries. Results are bit-identical to one launch per
+ * step. Every wait in the kernel is bounded; an expired wait makes the step
+ * call return GW_E_TIMEOUT. mode: 0 = one launch per step (the default: it is
+ * the faster form on MI355X, because a step of a small batch is bound by the
+ * close's serial arithmetic, which the launch-per-step form overlaps with the
+ * SSA and the launch queue, and the persistent form adds the exchange of the
+ * step's sums and constants through memory), 1 = persistent (GW_E_INVALID_ARG
+ * if the blocks cannot all be resident), 2 = persistent when they can and the
+ * launch shape is the default. Frozen replay always launches per chunk.
+ * gw_batch_get_persistent reports whether the next steps will run persistently.
+ */
+int gw_batch_set_persistent(gw_batch *b, int32_t mode, char *err, size_t errlen);
+int gw_batch_get_persistent(gw_batch *b, int32_t *active);
+/* Polls before a wait of the persistent kernel gives up (0 = the default, about 2
+ * million, seconds of waiting). A diagnostic and test hook: a tiny limit makes a step
+ * call end with GW_E_TIMEOUT, and every block of the kernel still exits normally. */
+int gw_batch_set_spin_limit(gw_batch *b, int64_t polls, char *err, size_t errlen);
 /* Advance every cell by nsteps Euler steps. */
 int gw_batch_step(gw_batch *b, int64_t nsteps, char *err, size_t errlen);
 /* Simulated time of the batch (accumulated t += dt, as src/pyGW.hpp:116): the clock of the

@@ -248,12 +248,19 @@ class OracleModel:
     def step(self, dt: float = 1e-3, istim: float = 0.0) -> int:
         return self.L.gwo_model_euler_step(self.h, dt, istim)
 
-    def run(self, dt: float, num_steps: int, istim=None, record_every: int = 0):
-        """run<PRNG> (src/pyGW.hpp:102-122). Returns dict of records (or {} if record_every == 0)."""
+    def run(self, dt: float, num_steps: int, istim=None, record_every: int = 0, fields=None):
+        """run<PRNG> (src/pyGW.hpp:102-122). Returns dict of records (or {} if record_every == 0);
+        `fields` limits the records to the named fields (all by default)."""
         ist = None if istim is None else np.ascontiguousarray(istim, dtype=np.float64)
         rec = GwRecord()
         out = {}
-        if record_every > 0:
+        if record_every > 0 and fields is not None:
+            R = -(-num_steps // record_every)
+            out = {k: np.zeros(R) for k in fields}
+            rec.num_records = R
+            for k, v in out.items():
+                setattr(rec, k, v.ctypes.data)
+        elif record_every > 0:
             R = -(-num_steps // record_every)
             n = self.n
             out = {
