@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
         "gw_ctx_destroy": (C.c_int, [P]),
         "gw_run": (C.c_int, [P, C.POINTER(Params), i32, f64, i64, P, i32, C.POINTER(Globals), C.POINTER(CruState),
                              u64, i32, i32, C.POINTER(Record)] + E),
+        "gw_run_noalloc": (C.c_int, [P, C.POINTER(Params), i32, f64, i64, P, i32, C.POINTER(Globals),
+                                     C.POINTER(CruState), u64, i32, i32, C.POINTER(Record)] + E),
         "gw_batch_create": (C.c_int, [P, i32, P, i32, i32, f64, C.POINTER(Globals), C.POINTER(P)] + E),
         "gw_batch_destroy": (C.c_int, [P]),
         "gw_batch_set_cru_state": (C.c_int, [P, i32, C.POINTER(CruState)] + E),
@@ -438,8 +440,10 @@ def _crus_struct(crus: dict | None):
 
 
 def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, istim, record_every: int,
-        globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0, precision: int = PREC_FP64) -> dict:
-    """gw_run: the run<PRNG> loop (src/pyGW.hpp:102-122) with ceil(num_steps / record_every) records."""
+        globals_: Globals | None, crus: dict | None, seed: int, prng: int = 0, precision: int = PREC_FP64,
+        noalloc: bool = False) -> dict:
+    """gw_run: the run<PRNG> loop (src/pyGW.hpp:102-122) with ceil(num_steps / record_every) records.
+    noalloc=True calls gw_run_noalloc instead: the same run with no device allocation."""
     R = -(-int(num_steps) // int(record_every))
     n = int(n_cru)
     out = {"t": np.zeros(R), "V": np.zeros(R), "m": np.zeros(R), "h": np.zeros(R), "j": np.zeros(R),
@@ -454,7 +458,8 @@ def run(ctx: Context, params: Params, n_cru: int, dt: float, num_steps: int, ist
         setattr(rec, k, v.ctypes.data)
     ist = None if istim is None else np.ascontiguousarray(istim, np.float64)
     cs, keep = _crus_struct(crus)
-    _call(lib().gw_run, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
+    fn = lib().gw_run_noalloc if noalloc else lib().gw_run
+    _call(fn, ctx.h, C.byref(params), n, dt, int(num_steps), _ptr(ist), int(record_every),
           C.byref(globals_) if globals_ is not None else None, C.byref(cs) if cs is not None else None,
           int(seed) & (2**64 - 1), prng, precision, C.byref(rec))
     del keep

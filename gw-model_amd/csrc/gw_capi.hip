@@ -27,6 +27,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -1510,9 +1511,20 @@ gw_globals_t default_globals()
 
 }  // namespace
 
+// Device memory of the allocation-free entry point (gw_run_noalloc): one
+// static array, which every GPU the library runs on holds until the process
+// ends. Host code never passes the array's name to a kernel or a copy: it
+// uses the address hipGetSymbolAddress gives on each device (gw_ctx_create).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
 struct gw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;          // this device's address of `scratch`
     std::vector<gw_batch *> batches;  // live batches; released by gw_ctx_destroy
 };
 
@@ -2161,8 +2173,21 @@ int gw_ctx_create(gw_ctx **out, int device, char *err, size_t errlen)
         return GW_E_NO_DEVICE;
     }
     HIP_TRY(hipSetDevice(device));
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                set_err(err, errlen, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+                return GW_E_HIP;
+            }
+            g_scratch_ptr[device] = ptr;
+        }
+    }
     gw_ctx *c = new gw_ctx;
     c->device = device;
+    c->scratch = device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         set_err(err, errlen, "hipStreamCreate failed");
@@ -3003,7 +3028,8 @@ namespace {
 template <class Record>
 int run_loop(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
              int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
-             int32_t precision, int64_t num_records, Record &&record, char *err, size_t errlen)
+             int32_t precision, int64_t num_records, Record &&record, char *err, size_t errlen,
+             bool noalloc = false)
 {
     if (!ctx || !p || n_cru <= 0 || num_steps < 0 || record_every <= 0) {
         set_err(err, errlen, "gw_run: invalid arguments (n_cru > 0, num_steps >= 0, record_every > 0 required)");
@@ -3016,7 +3042,17 @@ int run_loop(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_
     d.params = *p;
     d.protocol.kind = GW_PROTO_TABLE;
     gw_batch *B = nullptr;
-    int rc = gw_batch_create(ctx, 1, &d, precision, prng_id, dt, g0, &B, err, errlen);
+    int rc;
+    if (noalloc) {
+        if (!ctx->scratch) {
+            set_err(err, errlen, "gw_run_noalloc: no scratch address for device %d", ctx->device);
+            return GW_E_INVALID_ARG;
+        }
+        rc = batch_create(ctx, 1, &d, precision, prng_id, dt, g0, ctx->scratch, kScratchBytes,
+                          istim ? num_steps : 0, &B, err, errlen);
+    } else {
+        rc = gw_batch_create(ctx, 1, &d, precision, prng_id, dt, g0, &B, err, errlen);
+    }
     if (rc) return rc;
     gw_batch_set_mode(B, 0, 0);
     if (c0) {
@@ -3056,9 +3092,14 @@ int run_loop(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_
 
 extern "C" {
 
-int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
-           int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
-           int32_t precision, gw_record_t *out, char *err, size_t errlen)
+}  // extern "C"
+
+namespace {
+
+// gw_run and gw_run_noalloc: the same run, records in the full layout.
+int run_full(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
+             int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
+             int32_t precision, gw_record_t *out, char *err, size_t errlen, bool noalloc)
 {
     const int64_t n = n_cru;
     auto record = [&](gw_batch *B, int64_t k, double t) {
@@ -3083,7 +3124,27 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t 
         return gw_batch_get_cru_state(B, 0, &c, err, errlen);
     };
     return run_loop(ctx, p, n_cru, dt, num_steps, istim, record_every, g0, c0, seed, prng_id, precision,
-                    out ? out->num_records : 0, record, err, errlen);
+                    out ? out->num_records : 0, record, err, errlen, noalloc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps, const double *istim,
+           int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed, int32_t prng_id,
+           int32_t precision, gw_record_t *out, char *err, size_t errlen)
+{
+    return run_full(ctx, p, n_cru, dt, num_steps, istim, record_every, g0, c0, seed, prng_id, precision, out, err,
+                    errlen, false);
+}
+
+int gw_run_noalloc(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps,
+                   const double *istim, int32_t record_every, const gw_globals_t *g0, const gw_cru_state_t *c0,
+                   uint64_t seed, int32_t prng_id, int32_t precision, gw_record_t *out, char *err, size_t errlen)
+{
+    return run_full(ctx, p, n_cru, dt, num_steps, istim, record_every, g0, c0, seed, prng_id, precision, out, err,
+                    errlen, true);
 }
 
 int gw_run_reduced(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt, int64_t num_steps,

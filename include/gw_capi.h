@@ -265,6 +265,24 @@ int gw_run(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt,
            int32_t prng_id, int32_t precision, gw_record_t *out,
            char *err, size_t errlen);
 /*
+ * gw_run for callers that cannot allocate device memory (code that must not
+ * call hipMalloc, such as tight serving loops or harnesses that forbid it).
+ * No counterpart in the reference. Same arguments, semantics and results as
+ * gw_run, bit for bit, but the call allocates nothing on the device: the
+ * cell's state, block partials, stimulus table and staging are carved from a
+ * static 64 MiB array that each GPU the library runs on holds until the
+ * process ends (a 12,500-CRU cell fits with a stimulus table of up to about
+ * 7.5 million steps; a larger run is refused with GW_E_INVALID_ARG). Every
+ * call on one GPU uses the same array and leaves its contents behind; each
+ * call initialises what it reads. Calls that overlap on one GPU (from two
+ * threads or two contexts) race on the array and must not be made.
+ */
+int gw_run_noalloc(gw_ctx *ctx, const gw_params_t *p, int32_t n_cru, double dt,
+                   int64_t num_steps, const double *istim, int32_t record_every,
+                   const gw_globals_t *g0, const gw_cru_state_t *c0, uint64_t seed,
+                   int32_t prng_id, int32_t precision, gw_record_t *out,
+                   char *err, size_t errlen);
+/*
  * gw_run with its records in the reduced encoding (gw_record_reduced_t),
  * replacing run() followed by sim_dict_reduced_memory (src/serialization.py:11-39)
  * without the full snapshots ever leaving the device.
