@@ -20,10 +20,13 @@ with the reference's algorithm).
   cell_beats   cells * (simulated ms / 500 ms per beat) per second
   roofline     for the step kernel: algorithmic flops per launch (F_ITER per
                SSA iteration x iterations counted in-kernel) and algorithmic
-               state bytes per launch (bytes per CRU-step x CRUs), each divided
-               by the kernel's average duration (the timed region between its
-               two HIP events over its SSA launches, which run back to back);
-               against the vector peak of the form's precision and HBM.
+               state bytes per launch (the CRU state read and written once per
+               launch: bytes per CRU x CRUs, whether the launch runs one SSA
+               step or two), each divided by the kernel's average duration (the
+               timed region between its two HIP events over its SSA launches,
+               which run back to back); against the vector peak of the form's
+               precision and HBM. The FP32-local form runs two SSA steps per
+               launch by default (--steps-per-launch).
                `traffic` is the measured HBM bytes per launch from
                profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and WRITE_SIZE,
                calibrated on a copy of known size) when present.
@@ -108,8 +111,9 @@ def make_cells(capi, workload, cells, crus):
     return out, 500.0
 
 
-def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=None):
-    """Time `steps` Euler steps of the batch after `warmup` untimed ones."""
+def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=None, steps_per_launch=1):
+    """Time `steps` Euler steps of the batch after `warmup` untimed ones (steps_per_launch: 2 runs two
+    SSA steps per launch where the launch shape supports it)."""
     ctx = capi.Context(0)
     tune = None
     if launch is not None:   # a fixed launch shape instead of the autotuner's choice
@@ -121,6 +125,7 @@ def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=
         # the state it starts from after its trials).
         t = capi.Batch(ctx, cell_list, precision=precision)
         t.set_mode(frozen=False, digest=False)
+        t.set_steps_per_launch(steps_per_launch)
         t.step(warmup + steps // 2)
         (lanes, blk, cpl), trials = t.autotune(autotune)
         tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_lane": cpl, "crus_per_block": blk * cpl // lanes,
@@ -131,6 +136,7 @@ def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=
     def fresh():
         x = capi.Batch(ctx, cell_list, precision=precision)
         x.set_mode(frozen=False, digest=False)
+        x.set_steps_per_launch(steps_per_launch)
         if tune is not None:
             x.set_launch(tune["lanes_per_cru"], tune["block"], tune["crus_per_lane"])
             x.set_graph(tune["graph_steps"])
@@ -139,19 +145,27 @@ def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=
         return x
 
     # The timed region runs the steps as a caller would (plain launches or
-    # replayed graphs) between two HIP events on the batch's stream. The step
-    # launches run back to back (no host gap: the kernel trace of the same
-    # command shows them so), so the region's time over its SSA launches is the
-    # step kernel's average duration. Per-dispatch timing events are not used:
-    # each makes its kernel's completion write back the caches, which adds
-    # about 3 us to every kernel.
+    # replayed graphs) between two HIP events on the batch's stream. With one
+    # step per launch the step launches run back to back (no host gap: the
+    # kernel trace of the same command shows them so), so the region's time
+    # over its SSA launches is the step kernel's average duration, and no
+    # per-dispatch events are used (each makes its kernel's completion write
+    # back the caches, about 3 us per kernel). With two steps per launch a
+    # close kernel runs between the SSA launches, so each SSA dispatch also
+    # records its own events and the kernel's average comes from those.
     b = fresh()
+    spl = b.steps_per_launch()
     i0 = b.iterations()
-    ms, _, launches = b.time_steps(steps, per_kernel=False)
+    ms, ssa, launches = b.time_steps(steps, per_kernel=spl > 1)
     iters = b.iterations() - i0
-    ssa_ms = ms
+    ssa_ms = ssa if spl > 1 else ms
     if tune is not None:
-        tune["kernel_timing"] = f"HIP events around the {steps} timed steps ({launches} SSA launches and one close-only launch)"
+        tune["steps_per_launch"] = spl
+        tune["kernel_timing"] = (f"HIP events around the {steps} timed steps ({launches} SSA launches and one "
+                                 "close-only launch)" if spl == 1 else
+                                 f"HIP events of each of the {launches} SSA launches of 2 steps (the close "
+                                 "kernels between them excluded); ms_per_step from HIP events around the "
+                                 f"{steps} timed steps")
     g = b.get_globals()
     obs = b.obs()
     b.close()
@@ -163,12 +177,15 @@ def kernel_name(tune, f32):
     """The step kernel a launch shape runs (xoshiro256** instantiation)."""
     lanes, blk, cpl = (tune["lanes_per_cru"], tune["block"], tune["crus_per_lane"]) if tune else (4, 256, 1)
     if lanes == 1:
+        if tune and tune.get("steps_per_launch", 1) == 2:
+            return f"k_step_lane2<0, {blk}, {cpl}>"
         return f"k_step_lane<0, {blk}, {cpl}>"
     return f"k_step<0, false, {'float' if f32 else 'double'}, {blk}>"
 
 
 def measured_traffic(total_crus, form):
-    """HBM bytes per SSA launch from the committed PMC summary, scaled to this batch."""
+    """HBM bytes per SSA launch from the committed PMC summary, scaled to this batch
+    (form: fp64, fp32, or fp32_pair for two steps per launch)."""
     path = os.path.join(ROOT, "profiles", "k_ssa_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -178,7 +195,8 @@ def measured_traffic(total_crus, form):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals[f"hbm_bytes_per_cru_step_{form}"]) * total_crus
+        key = "hbm_bytes_per_cru_launch_fp32_pair" if form == "fp32_pair" else f"hbm_bytes_per_cru_step_{form}"
+        return float(vals[key]) * total_crus
     except (KeyError, ValueError):
         return None
 
@@ -258,13 +276,15 @@ def measure(capi, precision, args, total_crus):
     fixed = args.launch_fp32 if f32 else args.launch_fp64
     launch = tuple(int(x) for x in (fixed + "x1").split("x")[:3]) if fixed and args.autotune <= 0 else None
     ms, ssa_ms, launches, iters, g, obs, tune = gpu_bench(capi, cell_list, args.steps, advance_steps(args),
-                                                          precision=precision, autotune=args.autotune, launch=launch)
+                                                          precision=precision, autotune=args.autotune, launch=launch,
+                                                          steps_per_launch=args.steps_per_launch if f32 else 1)
+    pair = bool(tune and tune.get("steps_per_launch", 1) == 2)
     sim_ms = args.steps * 1e-3
     sec = ms / 1e3
     per_launch_s = ssa_ms / 1e3 / launches
     flops_per_launch = F_ITER * iters / launches
     achieved_tflops = flops_per_launch / per_launch_s / 1e12
-    bytes_per_launch = (STATE_BYTES_F32 if f32 else STATE_BYTES) * total_crus
+    bytes_per_launch = (STATE_BYTES_F32 if f32 else STATE_BYTES) * total_crus   # the state crosses HBM once per launch
     achieved_gbs = bytes_per_launch / per_launch_s / 1e9
     peak_f = PEAK_FP32_TFLOPS if f32 else PEAK_FP64_TFLOPS
     frac_f = achieved_tflops / peak_f
@@ -274,7 +294,7 @@ def measure(capi, precision, args, total_crus):
         "value": total_crus * sim_ms / sec,
         "ms_per_step": ms / args.steps,
         "cell_beats_per_s": len(cell_list) * (sim_ms / period) / sec,
-        "iterations_per_cru_step": iters / (launches * total_crus),
+        "iterations_per_cru_step": iters / (args.steps * total_crus),
         "mean_V_end": float(np.mean([x.V for x in g])),
         "launch": tune,
         "roofline": {
@@ -283,7 +303,7 @@ def measure(capi, precision, args, total_crus):
             "peak": peak_f if valu else PEAK_HBM_GBS,
             "unit": "TFLOP/s" if valu else "GB/s",
             "frac": max(frac_f, frac_b),
-            "traffic": measured_traffic(total_crus, "fp32" if f32 else "fp64"),
+            "traffic": measured_traffic(total_crus, ("fp32_pair" if pair else "fp32") if f32 else "fp64"),
             "kernel": kernel_name(tune, f32),
             "avg_launch_us": per_launch_s * 1e6,
             "valu": {"achieved_tflops": achieved_tflops, "peak": peak_f, "frac": frac_f,
@@ -319,6 +339,9 @@ def main(argv=None):
                          "autotuner's choice on the seeds64 batch")
     ap.add_argument("--launch-fp64", default="4x64", metavar="LxB",
                     help="FP64 launch shape; the default is the autotuner's choice on the seeds64 batch")
+    ap.add_argument("--steps-per-launch", type=int, choices=[1, 2], default=2,
+                    help="SSA steps per launch of the FP32-local form (2: each CRU advances two steps per launch "
+                         "with its state in registers; one-lane shapes of 64 threads)")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

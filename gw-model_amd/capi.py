@@ -158,6 +158,8 @@ def lib() -> C.CDLL:
         "gw_batch_set_graph": (C.c_int, [P, i32] + E),
         "gw_batch_set_persistent": (C.c_int, [P, i32] + E),
         "gw_batch_get_persistent": (C.c_int, [P, C.POINTER(i32)]),
+        "gw_batch_set_steps_per_launch": (C.c_int, [P, i32] + E),
+        "gw_batch_get_steps_per_launch": (C.c_int, [P, C.POINTER(i32)]),
         "gw_batch_set_spin_limit": (C.c_int, [P, i64] + E),
         "gw_batch_get_currents": (C.c_int, [P, P, P] + E),
         "gw_batch_autotune": (C.c_int, [P, i64, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -381,9 +383,19 @@ class Batch:
         _call(self.L.gw_batch_set_graph, self.h, int(steps_per_graph))
 
     def set_persistent(self, mode: int):
-        """Persistent stepping: 0 one launch per step, 1 one launch per call (refused if the
-        blocks cannot all be resident), 2 persistent when they can (default)."""
+        """Persistent stepping: 0 plain launches (the default), 1 one launch per call (refused if
+        the blocks cannot all be resident), 2 persistent when they can be."""
         _call(self.L.gw_batch_set_persistent, self.h, int(mode))
+
+    def set_steps_per_launch(self, steps: int):
+        """SSA steps per plain launch: 2 (the default; FP32-local one-lane shapes of 64 threads) or 1."""
+        _call(self.L.gw_batch_set_steps_per_launch, self.h, int(steps))
+
+    def steps_per_launch(self) -> int:
+        """The SSA steps per launch the next plain steps use."""
+        v = i32()
+        self.L.gw_batch_get_steps_per_launch(self.h, C.byref(v))
+        return v.value
 
     def currents(self) -> tuple[np.ndarray, np.ndarray]:
         """(ICaL, Ito2) of every cell by the post-hoc definitions of src/model.py:414-481, on the device."""

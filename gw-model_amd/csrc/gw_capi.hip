@@ -468,12 +468,16 @@ struct CloseLds {
     double py[GD_N - GD_POW0];
     double sx[SC_N];         // arguments, then values, of the next step's constants
     double lookahead[3];     // V, Cai, CaNSR of the next step
+    gw_params_t p;           // the cell's parameters and derived constants, staged for the
+    CellConsts c;            // serial part (which would otherwise wait on them load by load)
 };
 union CloseShared {
     Partial red[kWave];
     CloseLds L;
 };
 static_assert(sizeof(gw_globals_t) == 36 * sizeof(double), "globals are 36 doubles");
+static_assert(sizeof(CloseLds) <= sizeof(Partial) * kWave, "the close's working set fits the reduction slots");
+static_assert(sizeof(gw_params_t) % 8 == 0 && sizeof(CellConsts) % 8 == 0, "staged as 8-byte words");
 
 // Close step n = D.step of one cell: the exact sum of the cell's block
 // partials of step n, globals += dGlobals (GW.tpp:252-271), t += dt, the
@@ -494,6 +498,20 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     const DevCell &C = a.cells[cell];
     const int ks = static_cast<int>(D.step & 1);
     const int tid = threadIdx.x;
+    // the parameters and derived constants, loaded now and stored to LDS once the
+    // reduction slots are free
+    constexpr int kPW = sizeof(gw_params_t) / 8, kCW = sizeof(CellConsts) / 8;
+    constexpr int kStage = (kPW + kCW + NS - 1) / NS;
+    unsigned long long stage[kStage];
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+        const int w = tid + r * NS;
+        stage[r] = 0;
+        if (tid < NS) {
+            if (w < kPW) stage[r] = reinterpret_cast<const unsigned long long *>(&C.p)[w];
+            else if (w < kPW + kCW) stage[r] = reinterpret_cast<const unsigned long long *>(&C.c)[w - kPW];
+        }
+    }
     Partial mine;
     partial_zero(mine);
     if (tid < NS)
@@ -528,6 +546,14 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         return;
     }
     CloseLds &L = sh.L;
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+        const int w = tid + r * NS;
+        if (tid < NS) {
+            if (w < kPW) reinterpret_cast<unsigned long long *>(&L.p)[w] = stage[r];
+            else if (w < kPW + kCW) reinterpret_cast<unsigned long long *>(&L.c)[w - kPW] = stage[r];
+        }
+    }
     const FluxSums S{fx_to_double(tot.jlcc), fx_to_double(tot.jxfer), fx_to_double(tot.jtr), static_cast<double>(tot.clch)};
     const double jrel = fx_to_double(tot.jrel);
     const double t_new = t_old + a.dt;
@@ -543,8 +569,8 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     }
     __syncthreads();
     const gw_globals_t &g = *reinterpret_cast<const gw_globals_t *>(L.g);
-    const double VF_RT = g.V * C.c.F_RT;
-    if (tid == 0) gd_args(C.p, g, VF_RT, L.x, L.py);
+    const double VF_RT = g.V * L.c.F_RT;
+    if (tid == 0) gd_args(L.p, g, VF_RT, L.x, L.py);
     __syncthreads();
     if (tid < GD_N) {
         const double v = gd_eval(tid, L.x[tid], tid >= GD_POW0 ? L.py[tid - GD_POW0] : 0.0);
@@ -555,7 +581,7 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     gw_globals_t dg;
     if (tid == 0) {
         const double Istim = protocol_istim(C, t_new, step_new);
-        const int rc = gd_finish(C.p, C.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
+        const int rc = gd_finish(L.p, L.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
         if (rc != GW_OK) D.status = rc;
         D.Istim = Istim;
         D.dg = dg;
@@ -567,7 +593,7 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
             CaNSR += dg.CaNSR;
         }
         if (clamp) V = protocol_vclamp(C, t_new + a.dt);
-        sc_args(V, C.c.F_RT, L.sx);
+        sc_args(V, L.c.F_RT, L.sx);
         L.lookahead[0] = V;
         L.lookahead[1] = Cai;
         L.lookahead[2] = CaNSR;
@@ -581,7 +607,7 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     if (tid < 36) reinterpret_cast<double *>(&D.g)[tid] = L.g[tid];
     if (tid != 0) return;
     const int slot = static_cast<int>((step_new + 1) & 1);
-    step_consts_x(D.k[slot], C.p, C.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
+    step_consts_x(D.k[slot], L.p, L.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
                   static_cast<uint32_t>((step_new + 1) & 0xffff));
     step_consts_f32(D.kf[slot], D.k[slot]);
     D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
@@ -597,7 +623,7 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     // that launch runs the step and none runs a later one.
     if (D.status != 0) D.stop_step = step_new + 1;
     const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
-    obs_update(D.obs, V_old, g.V, t_old, a.dt, g.Cai, cur.ICaL, open_frac, jrel * C.c.CRU_factor);
+    obs_update(D.obs, V_old, g.V, t_old, a.dt, g.Cai, cur.ICaL, open_frac, jrel * L.c.CRU_factor);
     if (a.trace_every > 0 && (step_new % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
         double *row = a.trace + (D.trace_rows * a.ncells + cell) * 4;
         row[0] = g.V;
@@ -1147,6 +1173,26 @@ __device__ __forceinline__ void ssa_block_persist(const KArgs &a, int b, int64_t
     }
 }
 
+// After the close of step n: publish the constants of step n + 2 it formed
+// (the cell's parity slot (n + 2) & 1) and the cell's stop step, then raise
+// the cell's counter to n + 3. The counter moves after the other stores have
+// completed. A cell that had already stopped publishes its unchanged slot,
+// so that every waiter goes on (and finds the stop step). Every thread of the
+// block takes part.
+template <class R>
+__device__ __forceinline__ void publish_consts(CellSync &X, const CellDyn &D, int64_t n)
+{
+    using KT = typename LocalState<R>::Consts;
+    constexpr int kWords = sizeof(KT) / 8;
+    const int par = static_cast<int>((n + 2) & 1);
+    for (int t = threadIdx.x; t < kWords; t += blockDim.x)
+        st_coh(&X.k[par][t], reinterpret_cast<const unsigned long long *>(&LocalState<R>::consts(D, par))[t]);
+    if (threadIdx.x == 0) st_coh(&X.k[par][kXchgWords - 1], static_cast<unsigned long long>(D.stop_step));
+    stores_done();
+    __syncthreads();
+    if (threadIdx.x == 0) st_coh(&X.kready, static_cast<unsigned long long>(n + 3));
+}
+
 // Wave 0 of a close block waits until every partial of the cell for step n
 // carries the tag n + 1 (each lane polls its share, the wave votes). The
 // partials of a parity are cleared before the launch (k_sync_init), and
@@ -1181,8 +1227,6 @@ template <class R>
 __device__ __forceinline__ void close_persist(const KArgs &a, int cell, int64_t step0, int64_t nsteps, CellDyn &Dl,
                                               CloseShared &sh, int &leave)
 {
-    using KT = typename LocalState<R>::Consts;
-    constexpr int kWords = sizeof(KT) / 8;
     static_assert(sizeof(CellDyn) % 8 == 0, "the dynamic state is whole 8-byte words");
     constexpr int kDynWords = sizeof(CellDyn) / 8;
     const DevCell &C = a.cells[cell];
@@ -1201,14 +1245,7 @@ __device__ __forceinline__ void close_persist(const KArgs &a, int cell, int64_t 
         if (leave) break;
         close_cell<kWave, true>(a, cell, Dl, sh);
         __syncthreads();
-        // publish the constants of step n + 2 and the stop step, then the counter
-        const int par = static_cast<int>((n + 2) & 1);
-        for (int t = threadIdx.x; t < kWords; t += blockDim.x)
-            st_coh(&X.k[par][t], reinterpret_cast<const unsigned long long *>(&LocalState<R>::consts(Dl, par))[t]);
-        if (threadIdx.x == 0) st_coh(&X.k[par][kXchgWords - 1], static_cast<unsigned long long>(Dl.stop_step));
-        stores_done();
-        __syncthreads();
-        if (threadIdx.x == 0) st_coh(&X.kready, static_cast<unsigned long long>(n + 3));
+        publish_consts<R>(X, Dl, n);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kDynWords; t += blockDim.x) g[t] = l[t];
@@ -1233,6 +1270,148 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<R>) void k_persist(KArgs a, i
         return;
     }
     ssa_block_persist<PRNG, DIGEST, R>(a, bid - a.ncells, step0, nsteps, T, acc, K, stop, leave);
+}
+
+// ---------------------------------------------------------------------------
+// Two Euler steps per launch, one lane per CRU (FP32-local). A launch runs
+// SSA(m) and SSA(m + 1) for every CRU with its state held in registers
+// between them, so the state crosses HBM once per two steps. SSA(m) needs the
+// constants formed by the close of step m - 2 and SSA(m + 1) those of the
+// close of step m - 1; neither close can run before every SSA block of the
+// previous launch has written its partial, so both run just before the
+// launch, in their own kernel (k_close_cells, one block per cell, compiled
+// with the whole register file so that its serial arithmetic does not spill).
+// A lane that has finished step m of its CRU goes on to step m + 1 of the
+// same CRU (reading that step's constants from its LDS slot), and only then
+// stores the CRU and takes the next one of its wave's range. Each lane sums
+// the outputs of each step separately in finishing order; the per-wave sums
+// are added exactly into one block partial per step. The launch sequence of a
+// call (host, launch_steps_pair): SSA(step0), SSA(step0 + 1) with no closes
+// before them (the call starts with every step closed); then the closes of
+// the two steps before each later pair; an odd last step runs alone after one
+// close; a last close kernel closes the rest. A cell that fails in either
+// close stops as with one step per launch: SSA(s) runs while s < stop_step,
+// which the closes before the launch have set. With one CRU per lane (CPT =
+// 1) a lane's sums are its CRU's own outputs, so the results are bit-identical
+// to one step per launch; with refilled lanes the order of a lane's FP64 sums
+// depends on which CRUs it ran, which can change their rounding.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+struct PairLds {
+    StepConstsF K[2];       // constants of steps m and m + 1
+    LccTableF T[2];
+    BlockAcc acc[2];
+    LaneSums<BLOCK> Ls[2];
+};
+
+// The block's exact partial of one step, from its accumulators.
+__device__ __forceinline__ void write_partial(const KArgs &a, const BlockAcc &acc, int64_t index)
+{
+    Partial p;
+    p.jxfer = acc_fx(acc, 0);
+    p.jtr = acc_fx(acc, 1);
+    p.jlcc = acc_fx(acc, 2);
+    p.jrel = acc_fx(acc, 3);
+    p.iters = acc.ints[0];
+    p.open = static_cast<int64_t>(acc.ints[1]);
+    p.clch = static_cast<int64_t>(acc.ints[2]);
+    p.err = acc.ints[3] ? 1 : 0;
+    p.tag = 0;
+    a.partials[index] = p;
+}
+
+template <int PRNG, int BLOCK, int CPT>
+__global__ __launch_bounds__(BLOCK, 4) void k_step_lane2(KArgs a, int64_t m, int32_t nss)
+{
+    constexpr int kMap = shape_of_crus(BLOCK * CPT);
+    static_assert(crus_per_block(kMap) == BLOCK * CPT, "CRUs per block must be 16 << map");
+    constexpr int kWords = sizeof(StepConstsF) / 4;
+    __shared__ PairLds<BLOCK> P;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int cell = __builtin_amdgcn_readfirstlane(a.block_cell[b]);
+    const DevCell &C = a.cells[cell];
+    const CellDyn &D = a.dyn[cell];
+    const SsaParamsF &sp = C.spf;
+    for (int t = tid; t < 2 * kWords; t += BLOCK) {
+        const int s = t / kWords, w = t % kWords;
+        if (s < nss) reinterpret_cast<uint32_t *>(&P.K[s])[w] = reinterpret_cast<const uint32_t *>(&D.kf[(m + s) & 1])[w];
+    }
+    acc_zero(P.acc[0]);
+    acc_zero(P.acc[1]);
+    P.Ls[0].zero(tid);
+    P.Ls[1].zero(tid);
+    __syncthreads();
+    for (int s = 0; s < nss; ++s) load_lcc_table(P.K[s], P.T[s]);
+    __syncthreads();
+    const float dt = static_cast<float>(a.dt);
+    const int64_t stop = D.stop_step;   // set by the closes before this launch
+    const bool run0 = m < stop;
+    const bool run1 = nss > 1 && m + 1 < stop;
+    const int64_t w0 = static_cast<int64_t>(b - C.blk0[kMap]) * (BLOCK * CPT) + wave * (kWave * CPT);
+    const int64_t w1 = min(w0 + kWave * CPT, static_cast<int64_t>(C.n_cru));
+    int64_t i = w0 + lane;
+    int64_t next = w0 + kWave;   // first CRU of the range not yet taken (wave-uniform)
+    bool have = (i < w1) && run0;
+    int sidx = 0;                // the step (0: m, 1: m + 1) this lane's CRU is in
+    LaneState S;
+    if (have) {
+        lane_load(a, C.slot0 + i, S);
+        lane_begin(S, sp);
+    }
+    while (__ballot(have)) {
+        bool fin = false;
+        if (have) {
+            fin = lane_iterate<PRNG>(S, sp, P.K[sidx], P.T[sidx], dt);
+            if (fin) {
+                P.Ls[sidx].add(tid, lane_outputs(S));
+                if (sidx == 0 && run1) {   // the same CRU, next step
+                    const int changed = S.changed;   // the store after step m + 1 keeps step m's changes
+                    sidx = 1;
+                    lane_begin(S, sp);
+                    S.changed = changed;
+                    fin = false;
+                } else {
+                    sidx = 0;
+                    lane_store(a, C.slot0 + i, S);
+                }
+            }
+        }
+        if constexpr (CPT > 1) {
+            const uint64_t msk = __ballot(fin);
+            if (fin) {
+                const int rank = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(msk >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(msk), 0u)));
+                i = next + rank;
+                have = i < w1;
+                if (have) {
+                    lane_load(a, C.slot0 + i, S);
+                    lane_begin(S, sp);
+                }
+            }
+            next += __popcll(msk);
+        } else {
+            if (fin) have = false;
+        }
+    }
+    for (int s = 0; s < nss; ++s) lane_accumulate(P.acc[s], P.Ls[s].get(tid));
+    __syncthreads();
+    if (tid < nss) write_partial(a, P.acc[tid], ((m + tid) & 1) * a.nblocks + b);
+}
+
+// ncl consecutive closes of every cell, one block of 64 threads per cell (the
+// two-steps-per-launch sequence). One wave per block and no occupancy target,
+// so the close's serial arithmetic keeps its values in registers.
+__global__ __launch_bounds__(kWave, 1) void k_close_cells(KArgs a, int32_t ncl)
+{
+    __shared__ CloseShared sh;
+    const int cell = blockIdx.x;
+    for (int c = 0; c < ncl; ++c) {
+        close_cell<kWave>(a, cell, a.dyn[cell], sh);
+        __syncthreads();
+    }
 }
 
 // Graph bookkeeping: the step index a replayed chunk starts at.
@@ -1542,6 +1721,7 @@ struct gw_batch {
     int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
     int32_t persist_mode = 0;        // persistent step kernel: 0 off (default), 1 on, 2 when every block fits at once
     int32_t persist_fit = -1;        // whether every block of the persistent kernel fits at once (-1: not known)
+    int32_t steps_per_launch = 2;    // SSA steps per plain launch: 2 (the default; FP32-local one-lane shapes of 64 threads) or 1
     int32_t cfg = 2;                 // launch configuration (kConfigs), 4 x 256 by default
     // replayed graphs of step chunks (0 = plain launches)
     int32_t graph_steps = 0;
@@ -1914,14 +2094,111 @@ hipError_t launch_persist(gw_batch *B, int64_t step0, int64_t n)
     return e;
 }
 
+// Whether the batch's plain launches run two SSA steps each (k_step_lane2):
+// asked for, FP32-local free-running steps on a one-lane shape of 64 threads
+// (xoshiro256**, no digests, as every one-lane shape), not persistent, no
+// replayed graphs.
+bool use_pair(gw_batch *B)
+{
+    if (B->steps_per_launch != 2 || B->frozen || B->graph_steps > 0 || use_persist(B)) return false;
+    const LaunchCfg &L = kConfigs[B->cfg];
+    return B->precision == GW_PREC_FP32_LOCAL && L.lanes == 1 && L.threads == 64 && B->prng == GW_PRNG_XOSHIRO256SS;
+}
+
+// One launch of k_step_lane2: SSA(m .. m + nss - 1); with e0/e1 the dispatch
+// records its own start and end.
+template <class K>
+void dispatch_lane2(K kernel, gw_batch *B, int64_t m, int32_t nss, hipEvent_t e0, hipEvent_t e1)
+{
+    const dim3 grid(static_cast<unsigned>(B->nblocks)), block(64);
+    if (e0)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, B->ctx->stream, e0, e1, 0u, B->a, m, nss);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, B->ctx->stream, B->a, m, nss);
+}
+
+void launch_lane2(gw_batch *B, int64_t m, int32_t nss, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+{
+    constexpr int P = GW_PRNG_XOSHIRO256SS;
+    switch (kConfigs[B->cfg].cpl) {
+    case 1: dispatch_lane2(k_step_lane2<P, 64, 1>, B, m, nss, e0, e1); break;
+    case 2: dispatch_lane2(k_step_lane2<P, 64, 2>, B, m, nss, e0, e1); break;
+    case 4: dispatch_lane2(k_step_lane2<P, 64, 4>, B, m, nss, e0, e1); break;
+    default: dispatch_lane2(k_step_lane2<P, 64, 8>, B, m, nss, e0, e1); break;
+    }
+}
+
+void launch_closes(gw_batch *B, int32_t ncl)
+{
+    hipLaunchKernelGGL(k_close_cells, dim3(B->ncells), dim3(kWave), 0, B->ctx->stream, B->a, ncl);
+}
+
+// Steps step0 .. step0+n-1 two per launch: the first launch runs SSA(step0)
+// and SSA(step0+1) with no closes before it (the call starts with every step
+// closed); each later launch is preceded by the closes its SSA steps need (up
+// to two), an odd last step runs alone, and a last close kernel closes the
+// rest. Returns the number of SSA launches through *launches. With ssa_ms,
+// every SSA dispatch records its own start and end through a ring of event
+// pairs (the host waits for the dispatch that last used a pair before reusing
+// it, kRing launches back, so the queue never runs dry), and *ssa_ms is the
+// sum of their durations: the SSA kernel's time without the close kernels.
+hipError_t launch_steps_pair(gw_batch *B, int64_t step0, int64_t n, int64_t *launches, double *ssa_ms = nullptr)
+{
+    constexpr int kRing = 256;
+    std::vector<hipEvent_t> ev(ssa_ms ? 2 * kRing : 0, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto &x : ev)
+        if ((e = hipEventCreate(&x)) != hipSuccess) break;
+    double acc = 0.0;
+    auto harvest = [&](int r) {
+        float f = 0.f;
+        hipError_t h = hipEventSynchronize(ev[2 * r + 1]);
+        if (h == hipSuccess) h = hipEventElapsedTime(&f, ev[2 * r], ev[2 * r + 1]);
+        acc += f;
+        return h;
+    };
+    int64_t m = step0, c = step0, count = 0;   // next SSA step, first step not closed
+    const int64_t end = step0 + n;
+    while (e == hipSuccess && m < end) {
+        const int32_t nss = end - m >= 2 ? 2 : 1;
+        const int32_t ncl = static_cast<int32_t>(std::max<int64_t>(0, m + nss - 2 - c));   // closes up to m+nss-3
+        if (ncl > 0) launch_closes(B, ncl);
+        if (ssa_ms) {
+            const int r = static_cast<int>(count % kRing);
+            if (count >= kRing && (e = harvest(r)) != hipSuccess) break;
+            launch_lane2(B, m, nss, ev[2 * r], ev[2 * r + 1]);
+        } else {
+            launch_lane2(B, m, nss);
+        }
+        c += ncl;
+        m += nss;
+        ++count;
+    }
+    if (e == hipSuccess && c < end) launch_closes(B, static_cast<int32_t>(end - c));
+    for (int64_t k = std::max<int64_t>(0, count - kRing); ssa_ms && e == hipSuccess && k < count; ++k)
+        e = harvest(static_cast<int>(k % kRing));
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (ssa_ms) *ssa_ms = acc;
+    if (launches) *launches = count;
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
 // Steps step0 .. step0+n-1: n launches, launch k running SSA(step0+k) beside
 // close(step0+k-1), then one launch with the last closes. Between calls every
 // step is closed, so the cell state read by the host is that of step0+n.
 // With graphs enabled, the middle launches are replays of the captured chunk.
-hipError_t launch_steps(gw_batch *B, int64_t step0, int64_t n)
+hipError_t launch_steps(gw_batch *B, int64_t step0, int64_t n, int64_t *launches = nullptr)
 {
+    if (launches) *launches = 0;
     if (n <= 0) return hipSuccess;
-    if (use_persist(B)) return launch_persist(B, step0, n);
+    if (use_persist(B)) {
+        if (launches) *launches = 1;
+        return launch_persist(B, step0, n);
+    }
+    if (use_pair(B)) return launch_steps_pair(B, step0, n, launches);
+    if (launches) *launches = n;
     launch_one(B, step0, 0, nullptr, nullptr);
     int64_t k = 1;
     if (B->graph_steps > 0 && n - 1 >= B->graph_steps) {
@@ -2918,11 +3195,14 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, B->ctx->stream));
     const bool persist = use_persist(B);
-    if (ssa_ms && !persist) {
+    int64_t nlaunch = nsteps;   // SSA launches
+    if (ssa_ms && use_pair(B)) {   // two steps per launch, each SSA dispatch timed
+        HIP_TRY(launch_steps_pair(B, B->steps_done, nsteps, &nlaunch, ssa_ms));
+    } else if (ssa_ms && !persist) {   // one step per launch, each dispatch timed
         rc = launch_steps_timed(B, B->steps_done, nsteps, ssa_ms, err, errlen);
         if (rc) return rc;
     } else {
-        HIP_TRY(launch_steps(B, B->steps_done, nsteps));
+        HIP_TRY(launch_steps(B, B->steps_done, nsteps, &nlaunch));
     }
     HIP_TRY(hipGetLastError());
     B->steps_done += nsteps;
@@ -2934,7 +3214,7 @@ int gw_batch_time_steps(gw_batch *B, int64_t nsteps, double *ms, double *ssa_ms,
     (void)hipEventDestroy(e1);
     *ms = f;
     if (ssa_ms && persist) *ssa_ms = f;   // one launch runs every step
-    if (launches) *launches = nsteps;
+    if (launches) *launches = nlaunch;
     return check_status(B, err, errlen);
 }
 
@@ -2994,6 +3274,24 @@ int gw_batch_get_persistent(gw_batch *B, int32_t *active)
     if (!B || !active) return GW_E_INVALID_ARG;
     (void)hipSetDevice(B->ctx->device);
     *active = use_persist(B) ? 1 : 0;
+    return GW_OK;
+}
+
+int gw_batch_set_steps_per_launch(gw_batch *B, int32_t steps, char *err, size_t errlen)
+{
+    if (!B || steps < 1 || steps > 2) {
+        set_err(err, errlen, "gw_batch_set_steps_per_launch: steps must be 1 or 2");
+        return GW_E_INVALID_ARG;
+    }
+    B->steps_per_launch = steps;
+    return GW_OK;
+}
+
+int gw_batch_get_steps_per_launch(gw_batch *B, int32_t *steps)
+{
+    if (!B || !steps) return GW_E_INVALID_ARG;
+    (void)hipSetDevice(B->ctx->device);
+    *steps = use_pair(B) ? 2 : 1;
     return GW_OK;
 }
 

@@ -369,6 +369,20 @@ int gw_batch_set_graph(gw_batch *b, int32_t steps_per_graph, char *err, size_t e
  */
 int gw_batch_set_persistent(gw_batch *b, int32_t mode, char *err, size_t errlen);
 int gw_batch_get_persistent(gw_batch *b, int32_t *active);
+/*
+ * SSA steps per plain launch, 2 (the default) or 1. No counterpart in the reference (its OpenMP loop,
+ * src/includes/GW.tpp:223-233, runs one step at a time). With 2, each launch of the
+ * FP32-local one-lane shapes of 64 threads (lanes_per_cru 1, block 64) advances every
+ * CRU two steps with its state held in registers, so the CRU state crosses HBM once per
+ * two steps; the two closes the launch's SSA steps need run just before it, in a kernel
+ * of their own. Other shapes, FP64, frozen globals, replayed graphs and the persistent
+ * kernel keep one step per launch. With one CRU per lane the results are bit-identical
+ * to one step per launch; with refilled lanes they are deterministic, and the rounding
+ * of the FP64 flux sums can differ (their order follows which CRUs each lane ran).
+ * gw_batch_get_steps_per_launch reports what the next plain steps use.
+ */
+int gw_batch_set_steps_per_launch(gw_batch *b, int32_t steps, char *err, size_t errlen);
+int gw_batch_get_steps_per_launch(gw_batch *b, int32_t *steps);
 /* Polls before a wait of the persistent kernel gives up (0 = the default, about 2
  * million, seconds of waiting). A diagnostic and test hook: a tiny limit makes a step
  * call end with GW_E_TIMEOUT, and every block of the kernel still exits normally. */
@@ -436,9 +450,12 @@ int gw_batch_autotune(gw_batch *b, int64_t trial_steps, int32_t *lanes_out, int3
                       char *err, size_t errlen);
 /*
  * Time nsteps Euler steps with HIP events on the batch's stream: *ms is the
- * wall time of the whole sequence, *launches the number of SSA dispatches.
- * When ssa_ms is not null, every SSA dispatch also records its own start/end
- * events and *ssa_ms is the sum of their durations.
+ * wall time of the whole sequence, *launches the number of SSA dispatches (one
+ * per step, one per two steps with gw_batch_set_steps_per_launch(b, 2), one for
+ * the persistent kernel; close-only launches are not counted). When ssa_ms is
+ * not null, every SSA dispatch also records its own start/end events and *ssa_ms
+ * is the sum of their durations (with two steps per launch, the SSA kernel
+ * without the close kernels between its launches).
  */
 int gw_batch_time_steps(gw_batch *b, int64_t nsteps, double *ms, double *ssa_ms, int64_t *launches,
                         char *err, size_t errlen);

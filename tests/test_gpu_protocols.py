@@ -216,8 +216,9 @@ def test_autotune_leaves_the_state_unchanged(gwcapi, gpu_ctx):
                          [(0, (4, 256, 1), True), (0, (4, 64, 1), False), (1, (1, 128, 1), False), (1, (1, 64, 4), False)],
                          ids=["fp64-digest", "fp64-4x64", "fp32-1x128", "fp32-1x64x4"])
 def test_graph_replays_match_plain_launches(gwcapi, gpu_ctx, precision, launch, digest):
-    """Steps replayed as captured graphs give the same bits as plain launches,
-    over calls of several lengths with the host reading the state between them."""
+    """Steps replayed as captured graphs give the same bits as plain launches of
+    one step each (graphs replay one step per launch), over calls of several
+    lengths with the host reading the state between them."""
     say(f"graphs {precision} {launch}")
     pace = {"kind": gwcapi.PROTO_PACE, "period": 1000.0, "duration": 1.0, "amplitude": 100.0, "start": 0.0}
     cells = [gwcapi.make_cell(n, s, None, pace) for n, s in ((300, 5), (1000, 6))]
@@ -226,6 +227,7 @@ def test_graph_replays_match_plain_launches(gwcapi, gpu_ctx, precision, launch, 
         b = gwcapi.Batch(gpu_ctx, cells, precision=precision)
         b.set_mode(frozen=False, digest=digest)
         b.set_launch(*launch)
+        b.set_steps_per_launch(1)
         b.set_graph(graph)
         runs.append(b)
     for chunk in (1, 40, 17, 100, 3, 250):
