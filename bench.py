@@ -129,14 +129,14 @@ def gpu_bench(capi, cell_list, steps, warmup, precision=0, autotune=200, launch=
         t.step(warmup + steps // 2)
         (lanes, blk, cpl), trials = t.autotune(autotune)
         tune = {"lanes_per_cru": lanes, "block": blk, "crus_per_lane": cpl, "crus_per_block": blk * cpl // lanes,
-                "graph_steps": t.graph_steps, "trial_steps": autotune, "tuned_at_step": warmup + steps // 2,
-                "trial_ms_per_step": trials}
+                "graph_steps": t.graph_steps, "tuned_steps_per_launch": t.steps_per_launch(),
+                "trial_steps": autotune, "tuned_at_step": warmup + steps // 2, "trial_ms_per_step": trials}
         t.close()
 
     def fresh():
         x = capi.Batch(ctx, cell_list, precision=precision)
         x.set_mode(frozen=False, digest=False)
-        x.set_steps_per_launch(steps_per_launch)
+        x.set_steps_per_launch(tune.get("tuned_steps_per_launch", steps_per_launch) if tune else steps_per_launch)
         if tune is not None:
             x.set_launch(tune["lanes_per_cru"], tune["block"], tune["crus_per_lane"])
             x.set_graph(tune["graph_steps"])
@@ -264,6 +264,65 @@ def copy_calibration(nbytes, reps=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def alternans_run(capi, args):
+    """The FP32-local batch from rest through --alternans-beats whole beats with the
+    on-device APD90 tracker: per-cell APD90 of the last two beats and the
+    beat-to-beat APD90 differences, and the whole-run throughput. Prints one
+    JSON line (rank 0); progress goes to stderr every 50 simulated ms."""
+    cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
+    crus = sum(c.n_cru for c in cell_list)
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, cell_list, precision=capi.PREC_FP32_LOCAL)
+    b.set_mode(frozen=False, digest=False)
+    shape = tuple(int(x) for x in (args.launch_fp32 + "x1").split("x")[:3])
+    b.set_launch(*shape)
+    b.set_steps_per_launch(args.steps_per_launch)
+    total = int(round(args.alternans_beats * period / 1e-3))
+    chunk = 50_000
+    t0 = time.perf_counter()
+    done = 0
+    while done < total:
+        n = min(chunk, total - done)
+        b.step(n)
+        done += n
+        print(f"bench: {done * 1e-3:.0f} of {total * 1e-3:.0f} simulated ms", file=sys.stderr, flush=True)
+    sec = time.perf_counter() - t0
+    obs = b.obs()
+    b.close()
+    ctx.close()
+    beats = np.array([o["beats_done"] for o in obs])
+    last = np.array([o["apd90_last"] for o in obs])
+    prev = np.array([o["apd90_prev"] for o in obs])
+    both = (last > 0) & (prev > 0)
+    alt = np.abs(last - prev)[both]
+    mean_abs = np.array([o["alt_sum"] / o["alt_count"] for o in obs if o["alt_count"] > 0])
+
+    def q(x):
+        if len(x) == 0:
+            return None
+        p = np.percentile(x, [0, 5, 50, 95, 100])
+        return {"min": p[0], "p5": p[1], "median": p[2], "p95": p[3], "max": p[4], "mean": float(np.mean(x))}
+
+    return {
+        "metric": "per-cell beat-to-beat APD90 alternans, FP32-local, 1 GPU",
+        "config": {"workload": WORKLOADS[args.workload] + f", dt 1e-3 ms, {args.alternans_beats} beats from rest",
+                   "name": args.workload, "cells": len(cell_list), "crus": crus, "beat_ms": period,
+                   "beats": args.alternans_beats, "steps": total,
+                   "launch": {"lanes_per_cru": shape[0], "block": shape[1], "crus_per_lane": shape[2],
+                              "steps_per_launch": args.steps_per_launch}},
+        "wall_s": sec,
+        "cru_ms_per_s": crus * total * 1e-3 / sec,
+        "cell_beats_per_s": len(cell_list) * args.alternans_beats / sec,
+        "beats_done": {str(int(k)): int(np.sum(beats == k)) for k in np.unique(beats)},
+        "apd90_last_ms": q(last[last > 0]),
+        "apd90_prev_ms": q(prev[prev > 0]),
+        "alternans_last_two_beats_ms": q(alt),
+        "mean_abs_dapd90_ms": q(mean_abs),
+        "cells_alternans_over_5ms": int(np.sum(alt > 5.0)),
+        "data": "synthetic: seeded rest state, paced from rest",
+    }
+
+
 def advance_steps(args):
     """Untimed steps before the timed window: to --window-start-ms, or --warmup if that is further."""
     return max(int(args.warmup), int(round(args.window_start_ms / 1e-3)))
@@ -344,6 +403,9 @@ def main(argv=None):
                          "with its state in registers; one-lane shapes of 64 threads)")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
+    ap.add_argument("--alternans-beats", type=int, default=0, metavar="N",
+                    help="only run the FP32-local batch from rest through N whole beats and report the per-cell "
+                         "APD90 alternans from the on-device tracker (one JSON line), instead of the throughput bench")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -357,6 +419,11 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
+    if args.alternans_beats > 0:
+        out = alternans_run(capi, args)
+        if rank == 0:
+            print(json.dumps(out), flush=True)
+        return 0
     cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
     total_crus = sum(c.n_cru for c in cell_list)
     forms = {"fp32": capi.PREC_FP32_LOCAL, "fp64": capi.PREC_FP64}

@@ -2098,12 +2098,14 @@ hipError_t launch_persist(gw_batch *B, int64_t step0, int64_t n)
 // asked for, FP32-local free-running steps on a one-lane shape of 64 threads
 // (xoshiro256**, no digests, as every one-lane shape), not persistent, no
 // replayed graphs.
-bool use_pair(gw_batch *B)
+bool pair_capable(gw_batch *B)
 {
-    if (B->steps_per_launch != 2 || B->frozen || B->graph_steps > 0 || use_persist(B)) return false;
+    if (B->frozen || B->graph_steps > 0 || use_persist(B)) return false;
     const LaunchCfg &L = kConfigs[B->cfg];
     return B->precision == GW_PREC_FP32_LOCAL && L.lanes == 1 && L.threads == 64 && B->prng == GW_PRNG_XOSHIRO256SS;
 }
+
+bool use_pair(gw_batch *B) { return B->steps_per_launch == 2 && pair_capable(B); }
 
 // One launch of k_step_lane2: SSA(m .. m + nss - 1); with e0/e1 the dispatch
 // records its own start and end.
@@ -3139,6 +3141,32 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         he = copy_all(false);
         B->steps_done = steps0;
     }
+    // the fastest shape with the other number of SSA steps per launch, where
+    // it can run both (the CRU state kept in registers across two steps, or
+    // streamed through HBM every step)
+    const int32_t spl0 = B->steps_per_launch;
+    int32_t best_spl = spl0;
+    set_shape(B, best_cfg);
+    if (he == hipSuccess && pair_capable(B)) {
+        B->steps_per_launch = 3 - spl0;
+        if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) == hipSuccess) {
+            B->steps_done += kAutotuneWarm;
+            if ((he = hipEventRecord(e0, st)) == hipSuccess && (he = launch_steps(B, B->steps_done, trial_steps)) == hipSuccess &&
+                (he = hipEventRecord(e1, st)) == hipSuccess && (he = hipEventSynchronize(e1)) == hipSuccess) {
+                float f = 0.f;
+                if ((he = hipEventElapsedTime(&f, e0, e1)) == hipSuccess) {
+                    const double ms = f / static_cast<double>(trial_steps);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best_spl = B->steps_per_launch;
+                    }
+                }
+            }
+        }
+        if (he == hipSuccess) he = copy_all(false);
+        B->steps_done = steps0;
+        B->steps_per_launch = best_spl;
+    }
     // the fastest shape once more with its steps replayed as captured graphs
     const int32_t graph0 = B->graph_steps;
     int32_t best_graph = 0;
@@ -3164,6 +3192,7 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
         B->steps_done = steps0;
     }
     B->graph_steps = (he == hipSuccess) ? best_graph : graph0;
+    B->steps_per_launch = (he == hipSuccess) ? best_spl : spl0;
     if (he == hipSuccess) he = copy_all(false);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     B->steps_done = steps0;

@@ -436,10 +436,14 @@ int gw_batch_set_launch(gw_batch *b, int32_t lanes_per_cru, int32_t block, int32
 int gw_batch_get_launch(gw_batch *b, int32_t *lanes_per_cru, int32_t *block, int32_t *crus_per_lane);
 /*
  * Autotuner: times trial_steps Euler steps of the batch in every launch
- * configuration available to it, each trial starting from the same state (the
- * batch state is saved before and restored after), keeps the fastest, then
- * times that shape with its steps replayed as captured graphs of 64 steps and
- * keeps graphs if they are more than 2% faster (*graph_out: 64 or 0). The trials run from
+ * configuration available to it (the FP32-local one-lane shapes of 64 threads
+ * with the batch's steps per launch), each trial starting from the same state
+ * (the batch state is saved before and restored after), keeps the fastest;
+ * where that shape can run one or two SSA steps per launch, times it with the
+ * other choice too and keeps the faster (gw_batch_get_steps_per_launch); then
+ * times the shape with its steps replayed as captured graphs of 64 steps (one
+ * step per launch) and keeps graphs if they are more than 2% faster
+ * (*graph_out: 64 or 0). The trials run from
  * the batch's current state: tune on a state like the one the run will spend
  * its time in. ms_per_step[GW_LAUNCH_CONFIGS + 1] (may be NULL) receives the
  * time per step of each configuration (gw_launch_config order; -1 where not

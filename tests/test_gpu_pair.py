@@ -126,3 +126,23 @@ def test_two_steps_per_launch_applies_only_where_built(gwcapi, gpu_ctx):
     with pytest.raises(gwcapi.GwError):
         b.set_steps_per_launch(3)
     b.close()
+
+
+def test_autotune_restores_the_state_and_picks_the_steps_per_launch(gwcapi, gpu_ctx):
+    """The autotuner times every FP32-local shape (the one-lane shapes of 64
+    threads with two steps per launch), then its pick with the other number of
+    steps per launch where it has both, and leaves the batch's state as it
+    found it."""
+    cells = [gwcapi.make_cell(2000, 10 + c, None, _pace(gwcapi)) for c in range(4)]
+    a = gwcapi.Batch(gpu_ctx, cells, precision=gwcapi.PREC_FP32_LOCAL)
+    a.set_mode(frozen=False, digest=False)
+    a.step(300)
+    before = _snapshot(a, len(cells))
+    shape, ms = a.autotune(40)
+    spl = a.steps_per_launch()
+    say(f"  autotune chose {shape}, {spl} steps per launch: {ms}")
+    assert a.launch() == shape
+    assert spl == 1 or shape[:2] == (1, 64)
+    assert all(ms[k] is not None and ms[k] > 0 for k in ms if k.startswith("1x"))
+    _assert_same(_snapshot(a, len(cells)), before, "autotune")
+    a.close()
