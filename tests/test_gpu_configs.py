@@ -1,4 +1,9 @@
-"""The batched workloads at their full shapes.
+"""The workloads at their full shapes.
+
+* One cell of 1,250 CRUs, default parameters, a 1 ms stimulus at t = 0 and
+  500 ms of simulated time with seed 42 (the plain method's CPU-runnable
+  case): the FP64 form bit-identical to the oracle over the whole beat, every
+  record and the final CRU state.
 
 * 256 cells with CRU counts uniform in 2,000..12,500, PCaL, GKr and GKs scaled
   log-uniform in [0.5, 2], per-cell seeds, 1 Hz pacing, FP32-local with the
@@ -20,9 +25,31 @@ import pytest
 
 from oracle.oracle import OracleModel
 
-from .test_gpu_parity import assert_globals_equal, assert_states_equal, oracle_params, say
+from .test_gpu_parity import (STATE_KEYS, _pulse, assert_globals_equal, assert_states_equal, oracle_params, say,
+                               to_oracle_globals)
 
 pytestmark = pytest.mark.gpu
+
+
+def test_plain_cell_whole_beat_matches_the_oracle(gwcapi, gpu_ctx):
+    n, steps, rec = 1250, 500_000, 10_000
+    say(f"plain cell: {n} CRUs x {steps} steps against the oracle")
+    m = OracleModel(n, seed=42, kind="par")
+    m.sample_rest_state()
+    crus = m.state()
+    istim = _pulse(steps)
+    g0 = gwcapi.default_globals()
+    out_g = gwcapi.run(gpu_ctx, gwcapi.Params.default(), n, 1e-3, steps, istim, rec, g0, crus, seed=42)
+    say("  GPU done; oracle")
+    m2 = OracleModel(n, seed=42, kind="par")
+    m2.set_state(globals_=to_oracle_globals(g0), **{k: crus[k] for k in STATE_KEYS if k != "JLCC"})
+    out_o = m2.run(1e-3, steps, istim, record_every=rec)
+    assert out_g["t"].shape[0] == steps // rec
+    for k, v in out_o.items():
+        assert np.array_equal(out_g[k], v), k
+    v = out_g["V"]
+    say(f"  V: peak {v.max():.1f} mV, last {v[-1]:.2f} mV")
+    assert v.max() > 0.0 and v[-1] < -80.0   # an action potential, back near rest by 490 ms
 
 
 def _pace(gwcapi, period):

@@ -51,6 +51,14 @@ def test_two_steps_per_launch_bit_identical_with_one_cru_per_lane(gwcapi, gpu_ct
         done += chunk
         say(f"  two steps per launch, one CRU per lane: {done} steps")
         _assert_same(_snapshot(one, len(cells)), _snapshot(two, len(cells)), done)
+        if done == 571:   # the same mid-run change of one cell's globals and another's CRU state in both
+            for b in (one, two):
+                g = b.get_globals()[1]
+                g.Cai *= 1.5
+                b.set_globals(1, g)
+                st = b.get_cru_state(2)
+                st["CaJSR"] = st["CaJSR"] * 0.9
+                b.set_cru_state(2, **st)
     one.close()
     two.close()
 
