@@ -468,6 +468,7 @@ struct CloseLds {
     double py[GD_N - GD_POW0];
     double sx[SC_N];         // arguments, then values, of the next step's constants
     double lookahead[3];     // V, Cai, CaNSR of the next step
+    double ch[25];           // the IKr, Kv4.3 and Kv1.4 chain derivatives (gw_globals_t order)
     gw_params_t p;           // the cell's parameters and derived constants, staged for the
     CellConsts c;            // serial part (which would otherwise wait on them load by load)
 };
@@ -478,6 +479,10 @@ union CloseShared {
 static_assert(sizeof(gw_globals_t) == 36 * sizeof(double), "globals are 36 doubles");
 static_assert(sizeof(CloseLds) <= sizeof(Partial) * kWave, "the close's working set fits the reduction slots");
 static_assert(sizeof(gw_params_t) % 8 == 0 && sizeof(CellConsts) % 8 == 0, "staged as 8-byte words");
+static_assert(offsetof(gw_globals_t, Kv43) == offsetof(gw_globals_t, Kr) + 5 * sizeof(double) &&
+                  offsetof(gw_globals_t, Kv14) == offsetof(gw_globals_t, Kr) + 15 * sizeof(double) &&
+                  sizeof(gw_globals_t) == offsetof(gw_globals_t, Kr) + 25 * sizeof(double),
+              "the three chains are the last 25 globals, in Kr, Kv4.3, Kv1.4 order");
 
 // Close step n = D.step of one cell: the exact sum of the cell's block
 // partials of step n, globals += dGlobals (GW.tpp:252-271), t += dt, the
@@ -512,22 +517,37 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
             else if (w < kPW + kCW) stage[r] = reinterpret_cast<const unsigned long long *>(&C.c)[w - kPW];
         }
     }
+    // this thread's global and its increment, read now for globals_apply below
+    double g_in = 0.0, dg_in = 0.0;
+    if (tid < 36) {
+        g_in = reinterpret_cast<const double *>(&D.g)[tid];
+        dg_in = reinterpret_cast<const double *>(&D.dg)[tid];
+    }
     Partial mine;
     partial_zero(mine);
-    if (tid < NS)
-        for (int q = tid; q < C.nblk[a.shape]; q += NS) {
-            const Partial *src = a.partials + ks * a.nblocks + C.blk0[a.shape] + q;
-            if constexpr (COH) {
+    if (tid < NS) {
+        const Partial *src = a.partials + ks * a.nblocks + C.blk0[a.shape];
+        const int nb = C.nblk[a.shape];
+        if constexpr (COH) {
+            for (int q = tid; q < nb; q += NS) {
                 Partial p;
                 unsigned long long *pw = reinterpret_cast<unsigned long long *>(&p);
-                const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(src);
+                const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(src + q);
                 for (int t = 0; t < static_cast<int>(sizeof(Partial) / 8); ++t)
                     pw[t] = __hip_atomic_load(sw + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 partial_add(mine, p);
-            } else {
-                partial_add(mine, *src);
             }
+        } else {
+            // two partials in flight per thread (the sums are exact, so the order does not matter)
+            int q = tid;
+            for (; q + NS < nb; q += 2 * NS) {
+                const Partial p0 = src[q], p1 = src[q + NS];
+                partial_add(mine, p0);
+                partial_add(mine, p1);
+            }
+            if (q < nb) partial_add(mine, src[q]);
         }
+    }
     reduce_slots<NS>(sh.red, mine, tid, tid < NS);
     const Partial tot = sh.red[0];
     const int status = D.status;
@@ -559,11 +579,9 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     const double t_new = t_old + a.dt;
     // globals_apply (GW.tpp:252-271), one global per thread; V is held under a
     // clamp and then set to the clamp potential of the new time (prepare_step)
-    const double *gD = reinterpret_cast<const double *>(&D.g);
-    const double *dgD = reinterpret_cast<const double *>(&D.dg);
     if (tid < 36) {
-        double v = gD[tid];
-        if (!a.frozen && !(tid == 0 && clamp)) v += dgD[tid];
+        double v = g_in;
+        if (!a.frozen && !(tid == 0 && clamp)) v += dg_in;
         if (tid == 0 && clamp) v = protocol_vclamp(C, t_new);
         L.g[tid] = v;
     }
@@ -577,12 +595,22 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         L.x[tid] = v;   // each thread reads and then overwrites its own slot
     }
     __syncthreads();
+    // gd_finish: the currents, concentrations, V and gates on lane 0 of wave 0;
+    // the three K-channel chains on the block's second and third waves when it
+    // has them (they read only the parameters, the globals and the evaluated
+    // exponentials), on lane 0 otherwise
     Currents cur;
     gw_globals_t dg;
+    const int nw = static_cast<int>(blockDim.x) / kWave;
     if (tid == 0) {
         const double Istim = protocol_istim(C, t_new, step_new);
-        const int rc = gd_finish(L.p, L.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
+        const int rc = gd_finish_main(L.p, L.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
         if (rc != GW_OK) D.status = rc;
+        if (nw == 1) {
+            gd_chain_kr(L.p, L.x, g, a.dt, dg.Kr);
+            gd_chain_kv(L.p, L.x, g, a.dt, 0, dg.Kv14);
+            gd_chain_kv(L.p, L.x, g, a.dt, 1, dg.Kv43);
+        }
         D.Istim = Istim;
         D.dg = dg;
         // lookahead_consts: the potential and concentrations the step after starts from
@@ -597,8 +625,18 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         L.lookahead[0] = V;
         L.lookahead[1] = Cai;
         L.lookahead[2] = CaNSR;
+    } else if (tid == kWave) {
+        gd_chain_kv(L.p, L.x, g, a.dt, 0, L.ch + 15);   // Kv1.4
+        if (nw == 2) {
+            gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 5);
+            gd_chain_kr(L.p, L.x, g, a.dt, L.ch);
+        }
+    } else if (tid == 2 * kWave) {
+        gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 5);    // Kv4.3
+        gd_chain_kr(L.p, L.x, g, a.dt, L.ch);           // IKr
     }
     __syncthreads();
+    if (nw > 1 && tid < 25) reinterpret_cast<double *>(&D.dg.Kr[0])[tid] = L.ch[tid];
     if (tid < SC_N) {
         const double v = gw::exp(L.sx[tid]);
         L.sx[tid] = v;
@@ -1401,10 +1439,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_step_lane2(KArgs a, int64_t m, int
     if (tid < nss) write_partial(a, P.acc[tid], ((m + tid) & 1) * a.nblocks + b);
 }
 
-// ncl consecutive closes of every cell, one block of 64 threads per cell (the
-// two-steps-per-launch sequence). One wave per block and no occupancy target,
-// so the close's serial arithmetic keeps its values in registers.
-__global__ __launch_bounds__(kWave, 1) void k_close_cells(KArgs a, int32_t ncl)
+// ncl consecutive closes of every cell, one block of three waves per cell (the
+// two-steps-per-launch sequence): the second and third waves form the K-channel
+// chains beside the first. No occupancy target, so the close's serial
+// arithmetic keeps its values in registers.
+__global__ __launch_bounds__(3 * kWave, 1) void k_close_cells(KArgs a, int32_t ncl)
 {
     __shared__ CloseShared sh;
     const int cell = blockIdx.x;
@@ -2132,7 +2171,7 @@ void launch_lane2(gw_batch *B, int64_t m, int32_t nss, hipEvent_t e0 = nullptr, 
 
 void launch_closes(gw_batch *B, int32_t ncl)
 {
-    hipLaunchKernelGGL(k_close_cells, dim3(B->ncells), dim3(kWave), 0, B->ctx->stream, B->a, ncl);
+    hipLaunchKernelGGL(k_close_cells, dim3(B->ncells), dim3(3 * kWave), 0, B->ctx->stream, B->a, ncl);
 }
 
 // Steps step0 .. step0+n-1 two per launch: the first launch runs SSA(step0)

@@ -355,9 +355,10 @@ struct Currents {
     double INa, INab, INaCa, INaK, IKr, IKs, Ito1, Ito2, IK1, IKp, ICaL, ICab, IpCa, Jup, Jtr_tot, Jxfer_tot;
 };
 
-// The derivatives from the values X[slot] of gd_args' arguments; the one
-// exponential whose argument needs a logarithm (IK1) is evaluated here.
-GW_HD int gd_finish(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT, const double *X,
+// The derivatives from the values X[slot] of gd_args' arguments, except the
+// IKr and Kv chains (gd_chain_kr / gd_chain_kv below); the one exponential
+// whose argument needs a logarithm (IK1) is evaluated here.
+GW_HD int gd_finish_main(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT, const double *X,
                     double dt, double Istim, const FluxSums &S, gw_globals_t &dg, Currents &cur)
 {
     const double V = g.V;
@@ -440,22 +441,46 @@ GW_HD int gd_finish(const gw_params_t &p, const CellConsts &c, const gw_globals_
     dg.h = dt * (alpha_h(V, X) * (1.0 - g.h) - beta_h(V, X) * g.h);
     dg.j = dt * (alpha_j(V, X) * (1.0 - g.j) - beta_j(V, X) * g.j);
     dg.xKs = dt * (XKsinf(X) - g.xKs) * tauXKs_inv(V, X);
-    // GW.tpp:25-44, 64-160: the IKr and Kv chains
-    kr_chain(p, X, g.Kr, dt, dg.Kr);
-    {
+    return GW_OK;
+}
+
+// GW.tpp:25-44, 126-133: the IKr chain's derivatives.
+GW_HD void gd_chain_kr(const gw_params_t &p, const double *X, const gw_globals_t &g, double dt, double dKr[5])
+{
+    kr_chain(p, X, g.Kr, dt, dKr);
+}
+
+// GW.tpp:64-123, 135-160: the derivatives of the Kv1.4 (which == 0) or Kv4.3
+// (which == 1) chain.
+GW_HD void gd_chain_kv(const gw_params_t &p, const double *X, const gw_globals_t &g, double dt, int which,
+                       double dKv[10])
+{
+    if (which == 0) {
         const double aa14 = p.alphaa0Kv14 * X[GD_KV0];
-        const double aa43 = p.alphaa0Kv43 * X[GD_KV1];
         const double ai14 = p.alphai0Kv14 * X[GD_KV2];
-        const double ai43 = p.alphai0Kv43 * X[GD_KV3];
         const double ba14 = p.betaa0Kv14 * X[GD_KV4];
-        const double ba43 = p.betaa0Kv43 * X[GD_KV5];
         const double bi14 = p.betai0Kv14 * X[GD_KV6];
-        const double bi43 = p.betai0Kv43 * X[GD_KV7];
         kv_chain(aa14, ai14, ba14, bi14, p.f1Kv14, p.f2Kv14, p.f3Kv14, p.f4Kv14, p.b1Kv14, p.b2Kv14, p.b3Kv14,
-                 p.b4Kv14, g.Kv14, dt, dg.Kv14);
+                 p.b4Kv14, g.Kv14, dt, dKv);
+    } else {
+        const double aa43 = p.alphaa0Kv43 * X[GD_KV1];
+        const double ai43 = p.alphai0Kv43 * X[GD_KV3];
+        const double ba43 = p.betaa0Kv43 * X[GD_KV5];
+        const double bi43 = p.betai0Kv43 * X[GD_KV7];
         kv_chain(aa43, ai43, ba43, bi43, p.f1Kv43, p.f2Kv43, p.f3Kv43, p.f4Kv43, p.b1Kv43, p.b2Kv43, p.b3Kv43,
-                 p.b4Kv43, g.Kv43, dt, dg.Kv43);
+                 p.b4Kv43, g.Kv43, dt, dKv);
     }
+}
+
+// All the derivatives from the values X[slot] of gd_args' arguments.
+GW_HD int gd_finish(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT, const double *X,
+                    double dt, double Istim, const FluxSums &S, gw_globals_t &dg, Currents &cur)
+{
+    const int rc = gd_finish_main(p, c, g, VF_RT, X, dt, Istim, S, dg, cur);
+    if (rc != GW_OK) return rc;
+    gd_chain_kr(p, X, g, dt, dg.Kr);
+    gd_chain_kv(p, X, g, dt, 0, dg.Kv14);
+    gd_chain_kv(p, X, g, dt, 1, dg.Kv43);
     return GW_OK;
 }
 
