@@ -1439,11 +1439,13 @@ __global__ __launch_bounds__(BLOCK, 4) void k_step_lane2(KArgs a, int64_t m, int
     if (tid < nss) write_partial(a, P.acc[tid], ((m + tid) & 1) * a.nblocks + b);
 }
 
-// ncl consecutive closes of every cell, one block of three waves per cell (the
-// two-steps-per-launch sequence): the second and third waves form the K-channel
-// chains beside the first. No occupancy target, so the close's serial
-// arithmetic keeps its values in registers.
-__global__ __launch_bounds__(3 * kWave, 1) void k_close_cells(KArgs a, int32_t ncl)
+// ncl consecutive closes of every cell, one block of two waves per cell (the
+// two-steps-per-launch sequence): the second wave forms the K-channel chains
+// beside the first. No occupancy target, so the close's serial arithmetic
+// keeps its values in registers (up to 256 VGPRs: two waves per SIMD, so the
+// blocks of 1,024 cells are resident at once; a third wave per block would
+// not be).
+__global__ __launch_bounds__(2 * kWave, 1) void k_close_cells(KArgs a, int32_t ncl)
 {
     __shared__ CloseShared sh;
     const int cell = blockIdx.x;
@@ -2171,7 +2173,7 @@ void launch_lane2(gw_batch *B, int64_t m, int32_t nss, hipEvent_t e0 = nullptr, 
 
 void launch_closes(gw_batch *B, int32_t ncl)
 {
-    hipLaunchKernelGGL(k_close_cells, dim3(B->ncells), dim3(3 * kWave), 0, B->ctx->stream, B->a, ncl);
+    hipLaunchKernelGGL(k_close_cells, dim3(B->ncells), dim3(2 * kWave), 0, B->ctx->stream, B->a, ncl);
 }
 
 // Steps step0 .. step0+n-1 two per launch: the first launch runs SSA(step0)
