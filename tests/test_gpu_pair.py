@@ -154,3 +154,32 @@ def test_autotune_restores_the_state_and_picks_the_steps_per_launch(gwcapi, gpu_
     assert all(ms[k] is not None and ms[k] > 0 for k in ms if k.startswith("1x"))
     _assert_same(_snapshot(a, len(cells)), before, "autotune")
     a.close()
+
+
+def test_two_steps_per_launch_at_the_extremes_of_the_cell_size(gwcapi, gpu_ctx):
+    """Cells of 1, 65 and 2^20 CRUs in one batch: with one CRU per lane two
+    steps per launch give the same bits as one; with two and eight CRUs per
+    lane they are deterministic and keep every channel count valid."""
+    cells = [gwcapi.make_cell(n, 300 + k, None, _pace(gwcapi)) for k, n in enumerate((1, 65, 1 << 20))]
+    one = _batch(gwcapi, gpu_ctx, cells, (1, 64, 1), 1)
+    two = _batch(gwcapi, gpu_ctx, cells, (1, 64, 1), 2)
+    for chunk in (3, 500):
+        one.step(chunk)
+        two.step(chunk)
+    say("  extremes, one CRU per lane: 503 steps")
+    _assert_same(_snapshot(one, len(cells)), _snapshot(two, len(cells)), "extremes")
+    one.close()
+    two.close()
+    for cpl in (2, 8):
+        snaps = []
+        for run in range(2):
+            b = _batch(gwcapi, gpu_ctx, cells, (1, 64, cpl), 2)
+            b.step(1003)
+            snaps.append(_snapshot(b, len(cells)))
+            b.close()
+        _assert_same(snaps[0], snaps[1], f"run to run, {cpl} CRUs per lane")
+        for c, st in enumerate(snaps[0][1]):
+            R = st["RyR"]
+            assert np.all(R.sum(axis=2) == 5), (cpl, c)
+            assert np.all((st["LCC"] >= 1) & (st["LCC"] <= 12)), (cpl, c)
+        say(f"  extremes, {cpl} CRUs per lane: deterministic")
