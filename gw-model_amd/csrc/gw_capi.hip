@@ -468,7 +468,8 @@ struct CloseLds {
     double py[GD_N - GD_POW0];
     double sx[SC_N];         // arguments, then values, of the next step's constants
     double lookahead[3];     // V, Cai, CaNSR of the next step
-    double ch[25];           // the IKr, Kv4.3 and Kv1.4 chain derivatives (gw_globals_t order)
+    double ch[29];           // the derivatives of the gates m, h, j, xKs and of the IKr, Kv4.3 and
+                             // Kv1.4 chains (the last 29 globals, gw_globals_t order)
     gw_params_t p;           // the cell's parameters and derived constants, staged for the
     CellConsts c;            // serial part (which would otherwise wait on them load by load)
 };
@@ -479,10 +480,13 @@ union CloseShared {
 static_assert(sizeof(gw_globals_t) == 36 * sizeof(double), "globals are 36 doubles");
 static_assert(sizeof(CloseLds) <= sizeof(Partial) * kWave, "the close's working set fits the reduction slots");
 static_assert(sizeof(gw_params_t) % 8 == 0 && sizeof(CellConsts) % 8 == 0, "staged as 8-byte words");
-static_assert(offsetof(gw_globals_t, Kv43) == offsetof(gw_globals_t, Kr) + 5 * sizeof(double) &&
+static_assert(offsetof(gw_globals_t, h) == offsetof(gw_globals_t, m) + sizeof(double) &&
+                  offsetof(gw_globals_t, xKs) == offsetof(gw_globals_t, m) + 3 * sizeof(double) &&
+                  offsetof(gw_globals_t, Kr) == offsetof(gw_globals_t, m) + 4 * sizeof(double) &&
+                  offsetof(gw_globals_t, Kv43) == offsetof(gw_globals_t, Kr) + 5 * sizeof(double) &&
                   offsetof(gw_globals_t, Kv14) == offsetof(gw_globals_t, Kr) + 15 * sizeof(double) &&
-                  sizeof(gw_globals_t) == offsetof(gw_globals_t, Kr) + 25 * sizeof(double),
-              "the three chains are the last 25 globals, in Kr, Kv4.3, Kv1.4 order");
+                  sizeof(gw_globals_t) == offsetof(gw_globals_t, m) + 29 * sizeof(double),
+              "the gates and the three chains are the last 29 globals: m, h, j, xKs, Kr, Kv4.3, Kv1.4");
 
 // Close step n = D.step of one cell: the exact sum of the cell's block
 // partials of step n, globals += dGlobals (GW.tpp:252-271), t += dt, the
@@ -595,10 +599,10 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         L.x[tid] = v;   // each thread reads and then overwrites its own slot
     }
     __syncthreads();
-    // gd_finish: the currents, concentrations, V and gates on lane 0 of wave 0;
-    // the three K-channel chains on the block's second and third waves when it
-    // has them (they read only the parameters, the globals and the evaluated
-    // exponentials), on lane 0 otherwise
+    // gd_finish: the currents, concentrations and V on lane 0 of wave 0; the
+    // gates and the three K-channel chains on the block's second and third
+    // waves when it has them (they read only the parameters, the globals and
+    // the evaluated exponentials), on lane 0 otherwise
     Currents cur;
     gw_globals_t dg;
     const int nw = static_cast<int>(blockDim.x) / kWave;
@@ -607,6 +611,12 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         const int rc = gd_finish_main(L.p, L.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
         if (rc != GW_OK) D.status = rc;
         if (nw == 1) {
+            double gates[4];
+            gd_gates(L.x, g, a.dt, gates);
+            dg.m = gates[0];
+            dg.h = gates[1];
+            dg.j = gates[2];
+            dg.xKs = gates[3];
             gd_chain_kr(L.p, L.x, g, a.dt, dg.Kr);
             gd_chain_kv(L.p, L.x, g, a.dt, 0, dg.Kv14);
             gd_chain_kv(L.p, L.x, g, a.dt, 1, dg.Kv43);
@@ -626,17 +636,18 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         L.lookahead[1] = Cai;
         L.lookahead[2] = CaNSR;
     } else if (tid == kWave) {
-        gd_chain_kv(L.p, L.x, g, a.dt, 0, L.ch + 15);   // Kv1.4
+        gd_chain_kv(L.p, L.x, g, a.dt, 0, L.ch + 19);   // Kv1.4
+        gd_gates(L.x, g, a.dt, L.ch);                   // m, h, j, xKs
         if (nw == 2) {
-            gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 5);
-            gd_chain_kr(L.p, L.x, g, a.dt, L.ch);
+            gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 9);
+            gd_chain_kr(L.p, L.x, g, a.dt, L.ch + 4);
         }
     } else if (tid == 2 * kWave) {
-        gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 5);    // Kv4.3
-        gd_chain_kr(L.p, L.x, g, a.dt, L.ch);           // IKr
+        gd_chain_kv(L.p, L.x, g, a.dt, 1, L.ch + 9);    // Kv4.3
+        gd_chain_kr(L.p, L.x, g, a.dt, L.ch + 4);       // IKr
     }
     __syncthreads();
-    if (nw > 1 && tid < 25) reinterpret_cast<double *>(&D.dg.Kr[0])[tid] = L.ch[tid];
+    if (nw > 1 && tid < 29) reinterpret_cast<double *>(&D.dg.m)[tid] = L.ch[tid];
     if (tid < SC_N) {
         const double v = gw::exp(L.sx[tid]);
         L.sx[tid] = v;

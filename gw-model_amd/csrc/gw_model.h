@@ -356,8 +356,9 @@ struct Currents {
 };
 
 // The derivatives from the values X[slot] of gd_args' arguments, except the
-// IKr and Kv chains (gd_chain_kr / gd_chain_kv below); the one exponential
-// whose argument needs a logarithm (IK1) is evaluated here.
+// gates and the IKr and Kv chains (gd_gates, gd_chain_kr and gd_chain_kv
+// below); the one exponential whose argument needs a logarithm (IK1) is
+// evaluated here.
 GW_HD int gd_finish_main(const gw_params_t &p, const CellConsts &c, const gw_globals_t &g, double VF_RT, const double *X,
                     double dt, double Istim, const FluxSums &S, gw_globals_t &dg, Currents &cur)
 {
@@ -436,12 +437,18 @@ GW_HD int gd_finish_main(const gw_params_t &p, const CellConsts &c, const gw_glo
     dg.CaHTRPN *= dt;
     dg.V = dt * (Istim - (cur.INa + cur.ICaL + cur.IKr + cur.IKs + cur.Ito1 + cur.IK1 + cur.IKp + cur.Ito2 +
                           cur.INaK + cur.INaCa + cur.IpCa + cur.ICab + cur.INab));
-    // GW.tpp:162-168
-    dg.m = dt * (alpha_m(V, X) * (1.0 - g.m) - beta_m(X) * g.m);
-    dg.h = dt * (alpha_h(V, X) * (1.0 - g.h) - beta_h(V, X) * g.h);
-    dg.j = dt * (alpha_j(V, X) * (1.0 - g.j) - beta_j(V, X) * g.j);
-    dg.xKs = dt * (XKsinf(X) - g.xKs) * tauXKs_inv(V, X);
     return GW_OK;
+}
+
+// GW.tpp:162-168: the derivatives of the gates m, h, j and xKs, into
+// d[0..3] (gw_globals_t order).
+GW_HD void gd_gates(const double *X, const gw_globals_t &g, double dt, double d[4])
+{
+    const double V = g.V;
+    d[0] = dt * (alpha_m(V, X) * (1.0 - g.m) - beta_m(X) * g.m);
+    d[1] = dt * (alpha_h(V, X) * (1.0 - g.h) - beta_h(V, X) * g.h);
+    d[2] = dt * (alpha_j(V, X) * (1.0 - g.j) - beta_j(V, X) * g.j);
+    d[3] = dt * (XKsinf(X) - g.xKs) * tauXKs_inv(V, X);
 }
 
 // GW.tpp:25-44, 126-133: the IKr chain's derivatives.
@@ -478,6 +485,12 @@ GW_HD int gd_finish(const gw_params_t &p, const CellConsts &c, const gw_globals_
 {
     const int rc = gd_finish_main(p, c, g, VF_RT, X, dt, Istim, S, dg, cur);
     if (rc != GW_OK) return rc;
+    double gates[4];
+    gd_gates(X, g, dt, gates);
+    dg.m = gates[0];
+    dg.h = gates[1];
+    dg.j = gates[2];
+    dg.xKs = gates[3];
     gd_chain_kr(p, X, g, dt, dg.Kr);
     gd_chain_kv(p, X, g, dt, 0, dg.Kv14);
     gd_chain_kv(p, X, g, dt, 1, dg.Kv43);
