@@ -32,10 +32,10 @@ with the reference's algorithm).
                calibrated on a copy of known size) when present.
   cpu_baseline the CPU restatement built with the reference's release flags,
                OpenMP over CRUs on all host cores, on a bounded sample: the
-               batch's first cell (its parameters and seed) over the same
-               timed window; a one-thread sample of a smaller cell over the
-               same window beside it (per-core throughput), and the host CPU
-               model
+               batch's first two cells (their parameters and seeds), one after
+               another, over the same timed window; a one-thread sample of a
+               smaller cell over the same window beside it (per-core
+               throughput), and the host CPU model
   launch       the SSA launch shape: by default the autotuner's choice on
                this batch (1x64x4 FP32-local, 4x64 FP64), fixed so that a
                profile of the same command sees the same kernels;
@@ -384,6 +384,8 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--crus", type=int, default=12500)
     ap.add_argument("--cpu-crus", type=int, default=12500)
+    ap.add_argument("--cpu-cells", type=int, default=2,
+                    help="cells of the batch the CPU sample runs, one after another (their own parameters and seeds)")
     ap.add_argument("--cpu-steps", type=int, default=0, help="steps of the CPU sample (0: the timed window's)")
     ap.add_argument("--cpu-crus-1t", type=int, default=1500, help="CRUs of the one-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -467,16 +469,22 @@ def main(argv=None):
         line["speedup_fp32_local_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     value = head["value"]
     if not args.no_cpu and rank == 0:
-        # the batch's first cell (its parameters, seed and pacing) over the same window
-        c0 = cell_list[0]
+        # the batch's first cells (their parameters, seeds and pacing) over the same window
+        sample = cell_list[:max(1, args.cpu_cells)]
         cpu_steps = args.cpu_steps or args.steps
         adv = advance_steps(args)
-        v, threads, secs = cpu_baseline(args.cpu_crus, cpu_steps, adv, seed=c0.seed, params=c0.params, period=period)
+        secs, threads = 0.0, 0
+        for c in sample:
+            _, threads, dt_s = cpu_baseline(args.cpu_crus, cpu_steps, adv, seed=c.seed, params=c.params, period=period)
+            secs += dt_s
+        v = len(sample) * args.cpu_crus * cpu_steps * 1e-3 / secs
+        c0 = sample[0]
         v1, _, secs1 = cpu_baseline(args.cpu_crus_1t, cpu_steps, adv, seed=c0.seed, threads=1, params=c0.params,
                                     period=period)
         window = f"t = {adv * 1e-3:g}..{(adv + cpu_steps) * 1e-3:g} ms of beat 1 after {adv} untimed steps"
         line["cpu_baseline"] = {"value": v, "unit": "CRU\u00b7ms/s", "cores": threads, "kind": "port",
-                                "sample": f"the batch's first cell: 1 cell x {args.cpu_crus} CRUs x {cpu_steps} "
+                                "sample": f"the batch's first {len(sample)} cell{'s' if len(sample) > 1 else ''}, "
+                                          f"one after another: {len(sample)} x {args.cpu_crus} CRUs x {cpu_steps} "
                                           f"steps ({window}), {secs:.1f} s",
                                 "cpu_model": cpu_model(),
                                 "one_thread": {"value": v1, "sample": f"1 cell x {args.cpu_crus_1t} CRUs x "
