@@ -323,6 +323,67 @@ def alternans_run(capi, args):
     }
 
 
+def ensemble_run(capi, args):
+    """The seed ensemble in both forms: every cell from rest through --ensemble-beats
+    whole beats, the on-device observables reset at the start of the last beat, and
+    the last beat's APD90, peak Cai, peak I_CaL and mean RyR open fraction compared
+    seed by seed (mean paired difference with its 95% interval, relative to the FP64
+    mean) and as distributions (variance ratio, two-sample KS). One JSON line."""
+    from scipy import stats
+
+    cell_list, period = make_cells(capi, args.workload, args.cells, args.crus)
+    beat_steps = int(round(period / 1e-3))
+    shapes = {"fp64": tuple(int(x) for x in (args.launch_fp64 + "x1").split("x")[:3]),
+              "fp32": tuple(int(x) for x in (args.launch_fp32 + "x1").split("x")[:3])}
+    res, walls = {}, {}
+    for form, prec in (("fp64", capi.PREC_FP64), ("fp32", capi.PREC_FP32_LOCAL)):
+        ctx = capi.Context(0)
+        b = capi.Batch(ctx, cell_list, precision=prec)
+        b.set_mode(frozen=False, digest=False)
+        b.set_launch(*shapes[form])
+        b.set_steps_per_launch(args.steps_per_launch if form == "fp32" else 1)
+        t0 = time.perf_counter()
+        for beat in range(args.ensemble_beats):
+            if beat == args.ensemble_beats - 1:
+                b.reset_obs()
+            for k in range(0, beat_steps, 50_000):
+                b.step(min(50_000, beat_steps - k))
+                print(f"bench: {form} beat {beat + 1}, {(k + 50_000) * 1e-3:.0f} ms", file=sys.stderr, flush=True)
+        walls[form] = time.perf_counter() - t0
+        obs = b.obs()
+        b.close()
+        ctx.close()
+        res[form] = {"APD90": np.array([o["apd90_last"] for o in obs]),
+                     "peak_Cai": np.array([o["peak_Cai"] for o in obs]),
+                     "peak_ICaL": np.array([o["peak_ICaL_neg"] for o in obs]),
+                     "mean_open_RyR": np.array([o["sum_open_RyR_frac"] / o["n_steps"] for o in obs])}
+    out = {}
+    for k in res["fp64"]:
+        a, b_ = res["fp32"][k], res["fp64"][k]
+        ok = (a > -1) & (b_ > -1) if k == "APD90" else np.ones(len(a), bool)
+        a, b_ = a[ok], b_[ok]
+        d = a - b_
+        se = d.std(ddof=1) / np.sqrt(len(d))
+        ratio = a.var(ddof=1) / b_.var(ddof=1)
+        out[k] = {"seeds": int(len(d)), "fp64_mean": float(b_.mean()), "fp32_mean": float(a.mean()),
+                  "fp64_sd": float(b_.std(ddof=1)), "fp32_sd": float(a.std(ddof=1)),
+                  "paired_diff": float(d.mean()), "paired_diff_ci95": float(1.96 * se),
+                  "rel_error": float(d.mean() / b_.mean()), "rel_error_ci95": float(1.96 * se / abs(b_.mean())),
+                  "paired_diff_in_se": float(d.mean() / se) if se > 0 else 0.0,
+                  "variance_ratio": float(ratio),
+                  "variance_ratio_accept_1e-3": [float(stats.f.ppf(5e-4, len(a) - 1, len(b_) - 1)),
+                                                 float(stats.f.ppf(1 - 5e-4, len(a) - 1, len(b_) - 1))],
+                  "ks_p": float(stats.ks_2samp(a, b_).pvalue)}
+    crus = sum(c.n_cru for c in cell_list)
+    return {"metric": "seed-ensemble statistics of the last beat, FP32-local against FP64, 1 GPU",
+            "config": {"workload": (f"{len(cell_list)} independent seeds x {args.crus} CRUs, 1 Hz pacing"
+                                    if args.workload == "seeds64" else WORKLOADS[args.workload])
+                                   + f", dt 1e-3 ms, {args.ensemble_beats} beats from rest; observables of the last",
+                       "name": args.workload, "cells": len(cell_list), "crus": crus, "beat_ms": period,
+                       "beats": args.ensemble_beats, "launch": {k: list(v) for k, v in shapes.items()}},
+            "wall_s": walls, "last_beat": out, "data": "synthetic: seeded rest state, paced from rest"}
+
+
 def advance_steps(args):
     """Untimed steps before the timed window: to --window-start-ms, or --warmup if that is further."""
     return max(int(args.warmup), int(round(args.window_start_ms / 1e-3)))
@@ -405,6 +466,9 @@ def main(argv=None):
                          "with its state in registers; one-lane shapes of 64 threads)")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
+    ap.add_argument("--ensemble-beats", type=int, default=0, metavar="N",
+                    help="only run the batch in both forms from rest through N whole beats and compare the last "
+                         "beat's observables seed by seed (one JSON line), instead of the throughput bench")
     ap.add_argument("--alternans-beats", type=int, default=0, metavar="N",
                     help="only run the FP32-local batch from rest through N whole beats and report the per-cell "
                          "APD90 alternans from the on-device tracker (one JSON line), instead of the throughput bench")
@@ -421,6 +485,11 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
     capi = load_capi()
+    if args.ensemble_beats > 0:
+        out = ensemble_run(capi, args)
+        if rank == 0:
+            print(json.dumps(out), flush=True)
+        return 0
     if args.alternans_beats > 0:
         out = alternans_run(capi, args)
         if rank == 0:
