@@ -25,8 +25,8 @@ with the reference's algorithm).
                step or two), each divided by the kernel's average duration (the
                timed region between its two HIP events over its SSA launches,
                which run back to back); against the vector peak of the form's
-               precision and HBM. The FP32-local form runs two SSA steps per
-               launch by default (--steps-per-launch).
+               precision and HBM. On the default workload the FP32-local form
+               runs two SSA steps per launch (--steps-per-launch).
                `traffic` is the measured HBM bytes per launch from
                profiles/k_ssa_traffic.txt (rocprofv3 FETCH_SIZE and WRITE_SIZE,
                calibrated on a copy of known size) when present.
@@ -461,9 +461,10 @@ def main(argv=None):
                          "autotuner's choice on the seeds64 batch")
     ap.add_argument("--launch-fp64", default="4x64", metavar="LxB",
                     help="FP64 launch shape; the default is the autotuner's choice on the seeds64 batch")
-    ap.add_argument("--steps-per-launch", type=int, choices=[1, 2], default=2,
+    ap.add_argument("--steps-per-launch", type=int, choices=[0, 1, 2], default=0,
                     help="SSA steps per launch of the FP32-local form (2: each CRU advances two steps per launch "
-                         "with its state in registers; one-lane shapes of 64 threads)")
+                         "with its state in registers, one-lane shapes of 64 threads; 0, the default: the "
+                         "library's choice, two when a launch's blocks are more than the GPU holds at once)")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="forms to time; with both, value is the batched FP32-local form and fp64 sits beside it")
     ap.add_argument("--ensemble-beats", type=int, default=0, metavar="N",

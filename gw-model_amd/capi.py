@@ -388,7 +388,8 @@ class Batch:
         _call(self.L.gw_batch_set_persistent, self.h, int(mode))
 
     def set_steps_per_launch(self, steps: int):
-        """SSA steps per plain launch: 2 (the default; FP32-local one-lane shapes of 64 threads) or 1."""
+        """SSA steps per plain launch of the FP32-local one-lane shapes of 64 threads: 0 automatic (the
+        default: two when the launch's blocks are more than the GPU holds at once), 1 or 2."""
         _call(self.L.gw_batch_set_steps_per_launch, self.h, int(steps))
 
     def steps_per_launch(self) -> int:

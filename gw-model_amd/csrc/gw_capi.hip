@@ -1773,7 +1773,8 @@ struct gw_batch {
     int32_t shape = kShapeDefault;   // block -> cell map in use (16 << shape CRUs per block)
     int32_t persist_mode = 0;        // persistent step kernel: 0 off (default), 1 on, 2 when every block fits at once
     int32_t persist_fit = -1;        // whether every block of the persistent kernel fits at once (-1: not known)
-    int32_t steps_per_launch = 2;    // SSA steps per plain launch: 2 (the default; FP32-local one-lane shapes of 64 threads) or 1
+    int32_t steps_per_launch = 0;    // SSA steps per plain launch (FP32-local one-lane shapes of 64 threads): 1, 2, 0 automatic
+    int64_t pair_capacity[GW_LAUNCH_CONFIGS] = {};   // blocks of k_step_lane2 resident at once, per configuration (0: not known)
     int32_t cfg = 2;                 // launch configuration (kConfigs), 4 x 256 by default
     // replayed graphs of step chunks (0 = plain launches)
     int32_t graph_steps = 0;
@@ -2157,7 +2158,34 @@ bool pair_capable(gw_batch *B)
     return B->precision == GW_PREC_FP32_LOCAL && L.lanes == 1 && L.threads == 64 && B->prng == GW_PRNG_XOSHIRO256SS;
 }
 
-bool use_pair(gw_batch *B) { return B->steps_per_launch == 2 && pair_capable(B); }
+// How many blocks of the batch's two-step kernel the GPU holds at once.
+int64_t pair_capacity(gw_batch *B)
+{
+    int64_t &cap = B->pair_capacity[B->cfg];
+    if (cap > 0) return cap;
+    const void *fn = nullptr;
+    switch (kConfigs[B->cfg].cpl) {
+    case 1: fn = reinterpret_cast<const void *>(&k_step_lane2<GW_PRNG_XOSHIRO256SS, 64, 1>); break;
+    case 2: fn = reinterpret_cast<const void *>(&k_step_lane2<GW_PRNG_XOSHIRO256SS, 64, 2>); break;
+    case 4: fn = reinterpret_cast<const void *>(&k_step_lane2<GW_PRNG_XOSHIRO256SS, 64, 4>); break;
+    default: fn = reinterpret_cast<const void *>(&k_step_lane2<GW_PRNG_XOSHIRO256SS, 64, 8>); break;
+    }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->ctx->device) == hipSuccess)
+        cap = static_cast<int64_t>(per_cu) * cus;
+    return cap > 0 ? cap : INT64_MAX;
+}
+
+// Two steps per launch when asked for (2), or automatically (0) when the SSA
+// blocks of a launch are more than the GPU holds at once: then the state's
+// HBM traffic binds and halving it pays for the separate close kernel; a batch
+// whose blocks all fit at once is bound by latency, which that kernel adds to.
+bool use_pair(gw_batch *B)
+{
+    if (B->steps_per_launch == 1 || !pair_capable(B)) return false;
+    return B->steps_per_launch == 2 || B->nblocks > pair_capacity(B);
+}
 
 // One launch of k_step_lane2: SSA(m .. m + nss - 1); with e0/e1 the dispatch
 // records its own start and end.
@@ -3200,7 +3228,9 @@ int gw_batch_autotune(gw_batch *B, int64_t trial_steps, int32_t *lanes_out, int3
     int32_t best_spl = spl0;
     set_shape(B, best_cfg);
     if (he == hipSuccess && pair_capable(B)) {
-        B->steps_per_launch = 3 - spl0;
+        const int32_t used = use_pair(B) ? 2 : 1;   // the choice the trials above ran with
+        best_spl = used;
+        B->steps_per_launch = 3 - used;
         if ((he = launch_steps(B, B->steps_done, kAutotuneWarm)) == hipSuccess) {
             B->steps_done += kAutotuneWarm;
             if ((he = hipEventRecord(e0, st)) == hipSuccess && (he = launch_steps(B, B->steps_done, trial_steps)) == hipSuccess &&
@@ -3360,8 +3390,8 @@ int gw_batch_get_persistent(gw_batch *B, int32_t *active)
 
 int gw_batch_set_steps_per_launch(gw_batch *B, int32_t steps, char *err, size_t errlen)
 {
-    if (!B || steps < 1 || steps > 2) {
-        set_err(err, errlen, "gw_batch_set_steps_per_launch: steps must be 1 or 2");
+    if (!B || steps < 0 || steps > 2) {
+        set_err(err, errlen, "gw_batch_set_steps_per_launch: steps must be 1, 2 or 0 (automatic)");
         return GW_E_INVALID_ARG;
     }
     B->steps_per_launch = steps;

@@ -370,7 +370,7 @@ int gw_batch_set_graph(gw_batch *b, int32_t steps_per_graph, char *err, size_t e
 int gw_batch_set_persistent(gw_batch *b, int32_t mode, char *err, size_t errlen);
 int gw_batch_get_persistent(gw_batch *b, int32_t *active);
 /*
- * SSA steps per plain launch, 2 (the default) or 1. No counterpart in the reference (its OpenMP loop,
+ * SSA steps per plain launch: 0 (automatic, the default), 1 or 2. No counterpart in the reference (its OpenMP loop,
  * src/includes/GW.tpp:223-233, runs one step at a time). With 2, each launch of the
  * FP32-local one-lane shapes of 64 threads (lanes_per_cru 1, block 64) advances every
  * CRU two steps with its state held in registers, so the CRU state crosses HBM once per
@@ -379,7 +379,10 @@ int gw_batch_get_persistent(gw_batch *b, int32_t *active);
  * kernel keep one step per launch. With one CRU per lane the results are bit-identical
  * to one step per launch; with refilled lanes they are deterministic, and the rounding
  * of the FP64 flux sums can differ (their order follows which CRUs each lane ran).
- * gw_batch_get_steps_per_launch reports what the next plain steps use.
+ * Automatic: two steps per launch when a launch's SSA blocks are more than the GPU
+ * holds at once (the state's HBM traffic binds), one otherwise (latency binds, and the
+ * separate close kernel would add to it). gw_batch_get_steps_per_launch reports what
+ * the next plain steps use.
  */
 int gw_batch_set_steps_per_launch(gw_batch *b, int32_t steps, char *err, size_t errlen);
 int gw_batch_get_steps_per_launch(gw_batch *b, int32_t *steps);

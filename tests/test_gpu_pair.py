@@ -183,3 +183,24 @@ def test_two_steps_per_launch_at_the_extremes_of_the_cell_size(gwcapi, gpu_ctx):
             assert np.all(R.sum(axis=2) == 5), (cpl, c)
             assert np.all((st["LCC"] >= 1) & (st["LCC"] <= 12)), (cpl, c)
         say(f"  extremes, {cpl} CRUs per lane: deterministic")
+
+
+def test_steps_per_launch_automatic(gwcapi, gpu_ctx):
+    """By default (0, automatic) a batch runs two steps per launch when its SSA
+    blocks are more than the GPU holds at once, one otherwise."""
+    small = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(12500, 7 + c, None, _pace(gwcapi)) for c in range(4)],
+                         precision=gwcapi.PREC_FP32_LOCAL)
+    small.set_launch(1, 64, 4)
+    assert small.steps_per_launch() == 1          # 196 blocks: one round
+    small.close()
+    big = gwcapi.Batch(gpu_ctx, [gwcapi.make_cell(12500, 7 + c, None, _pace(gwcapi)) for c in range(100)],
+                       precision=gwcapi.PREC_FP32_LOCAL)
+    big.set_mode(frozen=False, digest=False)
+    big.set_launch(1, 64, 4)
+    assert big.steps_per_launch() == 2            # 4,900 blocks of 256 CRUs: more than one round
+    big.step(3)
+    big.set_steps_per_launch(1)
+    assert big.steps_per_launch() == 1
+    big.set_steps_per_launch(0)
+    assert big.steps_per_launch() == 2
+    big.close()
