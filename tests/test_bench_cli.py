@@ -97,3 +97,13 @@ def test_cpu_baseline_runs_the_batch_first_cell_over_the_window(bench):
     assert out.returncode == 0, out.stderr
     v, threads, secs = json.loads(out.stdout.strip().splitlines()[-1])
     assert v > 0 and threads >= 1 and secs > 0
+
+
+def test_whole_beat_modes_and_steps_per_launch_parse_before_the_gpu(bench):
+    """The whole-beat modes and the steps-per-launch choice are accepted
+    options; more than one GPU is still refused before any GPU work."""
+    for extra in (["--ensemble-beats", "5", "--workload", "seeds64"], ["--alternans-beats", "3"],
+                  ["--steps-per-launch", "0"], ["--steps-per-launch", "2"], ["--steps-per-launch", "1"]):
+        assert bench.main(["--gpus", "2"] + extra) == 2
+    with pytest.raises(SystemExit):
+        bench.main(["--steps-per-launch", "3"])
