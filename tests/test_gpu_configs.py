@@ -8,10 +8,12 @@
 * 256 cells with CRU counts uniform in 2,000..12,500, PCaL, GKr and GKs scaled
   log-uniform in [0.5, 2], per-cell seeds, 1 Hz pacing, FP32-local with the
   steps replayed as captured graphs and the launch shape picked by the
-  autotuner: through one beat (455 ms) every cell fires; the cells with a
-  completed beat are back at rest, the others (more L-type current) are still
-  in their action potential; channel counts stay valid; the same batch in FP64
-  gives its smallest cells bit-identical to the oracle.
+  autotuner, through its two beats: at 455 ms every cell has fired, the cells
+  with a completed beat are back at rest and the others (more L-type current)
+  are still in their action potential; at 2 s every cell that completed its
+  first beat by 455 ms has completed its second and is back at rest; channel
+  counts stay valid; the same batch in FP64 gives its smallest cells
+  bit-identical to the oracle.
 * 1,024 cells x 12,500 CRUs (12.8 M CRUs) with six conductances scaled
   log-uniform in [0.5, 2], 2 Hz pacing: FP32-local through the upstroke and the
   Ca release (50 ms) with every cell depolarised and releasing and the
@@ -90,7 +92,7 @@ def _batch256(gwcapi):
     return cells
 
 
-def test_uneven_256_cell_batch_through_one_beat(gwcapi, gpu_ctx):
+def test_uneven_256_cell_batch_through_two_beats(gwcapi, gpu_ctx):
     cells = _batch256(gwcapi)
     sizes = np.array([c.n_cru for c in cells])
     say(f"256-cell batch: {sizes.sum()} CRUs, {sizes.min()}..{sizes.max()} per cell")
@@ -125,6 +127,22 @@ def test_uneven_256_cell_batch_through_one_beat(gwcapi, gpu_ctx):
     # model the plateau is held by I_CaL, and with PCaL scaled down the dome is lost (APD90 below 150 ms)
     assert np.median(pcal[~done]) > 1.2 * np.median(pcal[done])
     assert np.median(pcal[done & (apd < 150)]) < 0.8
+    for c in np.argsort(sizes)[:4].tolist() + np.argsort(sizes)[-4:].tolist():
+        _check_counts(b.get_cru_state(c), c)
+    # the second beat: on to t = 2 s, just before the third stimulus
+    for k in range(31):
+        b.step(50_000 if k < 30 else 45_000)
+        if k % 5 == 4:
+            say(f"  t = {455 + min(k + 1, 30) * 50 + (45 if k == 30 else 0)} ms")
+    obs = b.obs()
+    beats = np.array([o["beats_done"] for o in obs])
+    apd2 = np.array([o["apd90_last"] for o in obs])
+    v2 = np.array([g.V for g in b.get_globals()])
+    say(f"  at 2 s: beats completed {dict(zip(*np.unique(beats, return_counts=True)))}; APD90 of the second beat "
+        f"of the first beat's repolarised cells {apd2[done].min():.1f}..{apd2[done].max():.1f} ms")
+    assert np.all(beats[done] == 2)                        # a normal first beat is followed by a second
+    assert np.all((apd2[done] > 30) & (apd2[done] < 1000))
+    assert np.all(v2[beats == 2] < -70.0)                  # and the cell is back at rest before the third
     for c in np.argsort(sizes)[:4].tolist() + np.argsort(sizes)[-4:].tolist():
         _check_counts(b.get_cru_state(c), c)
     b.close()
