@@ -178,17 +178,26 @@ __device__ void partial_add(Partial &a, const Partial &b)
     a.err |= b.err;
 }
 
-// Exact tree reduction of NS partials in LDS: the `writer` threads store
-// theirs at red[slot] (together they write every slot 0..NS-1), then the
-// first NS/2 threads fold the tree. All threads of the block call it.
-template <int NS>
-__device__ void reduce_slots(Partial *red, const Partial &mine, int slot, bool writer)
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
-    if (writer) red[slot] = mine;
-    __syncthreads();
-    for (int s = NS / 2; s > 0; s >>= 1) {
-        if (static_cast<int>(threadIdx.x) < s) partial_add(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
+    return static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(v), m));
+}
+
+// Exact sum of a partial over the 64 lanes of the calling wave, left in every
+// lane (a butterfly of lane exchanges). Every lane of the wave must call it.
+__device__ void partial_wave_sum(Partial &p)
+{
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        Partial o;
+        o.jlcc.lo = shfl_xor_u64(p.jlcc.lo, m);   o.jlcc.hi = shfl_xor_u64(p.jlcc.hi, m);
+        o.jxfer.lo = shfl_xor_u64(p.jxfer.lo, m); o.jxfer.hi = shfl_xor_u64(p.jxfer.hi, m);
+        o.jtr.lo = shfl_xor_u64(p.jtr.lo, m);     o.jtr.hi = shfl_xor_u64(p.jtr.hi, m);
+        o.jrel.lo = shfl_xor_u64(p.jrel.lo, m);   o.jrel.hi = shfl_xor_u64(p.jrel.hi, m);
+        o.clch = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(p.clch), m));
+        o.open = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(p.open), m));
+        o.iters = shfl_xor_u64(p.iters, m);
+        o.err = __shfl_xor(p.err, m);
+        partial_add(p, o);
     }
 }
 
@@ -460,6 +469,8 @@ __device__ __forceinline__ void ssa_block(const KArgs &a, int b, int64_t step0, 
     }
 }
 
+constexpr int kGd0 = 7;   // V, Nai, Ki, Cai, CaNSR, CaLTRPN, CaHTRPN: the globals gd_finish_main derives
+
 // LDS of a close block. The working set of the close reuses the slots of the
 // partial-sum reduction once its total has been read.
 struct CloseLds {
@@ -470,8 +481,12 @@ struct CloseLds {
     double lookahead[3];     // V, Cai, CaNSR of the next step
     double ch[29];           // the derivatives of the gates m, h, j, xKs and of the IKr, Kv4.3 and
                              // Kv1.4 chains (the last 29 globals, gw_globals_t order)
-    gw_params_t p;           // the cell's parameters and derived constants, staged for the
-    CellConsts c;            // serial part (which would otherwise wait on them load by load)
+    double dg0[kGd0];        // the derivatives of V and the concentrations (the first 7 globals)
+    double Istim;            // the stimulus of the new time
+    double ICaL;             // I_CaL of the close, for the observables
+    gw_params_t p;           // the cell's parameters, derived constants and observables,
+    CellConsts c;            // staged for the serial part (which would otherwise wait on
+    gw_cell_obs_t obs;       // them load by load)
 };
 union CloseShared {
     Partial red[kWave];
@@ -485,7 +500,8 @@ static_assert(offsetof(gw_globals_t, h) == offsetof(gw_globals_t, m) + sizeof(do
                   offsetof(gw_globals_t, Kr) == offsetof(gw_globals_t, m) + 4 * sizeof(double) &&
                   offsetof(gw_globals_t, Kv43) == offsetof(gw_globals_t, Kr) + 5 * sizeof(double) &&
                   offsetof(gw_globals_t, Kv14) == offsetof(gw_globals_t, Kr) + 15 * sizeof(double) &&
-                  sizeof(gw_globals_t) == offsetof(gw_globals_t, m) + 29 * sizeof(double),
+                  sizeof(gw_globals_t) == offsetof(gw_globals_t, m) + 29 * sizeof(double) &&
+                  offsetof(gw_globals_t, m) == kGd0 * sizeof(double),
               "the gates and the three chains are the last 29 globals: m, h, j, xKs, Kr, Kv4.3, Kv1.4");
 
 // Close step n = D.step of one cell: the exact sum of the cell's block
@@ -505,12 +521,25 @@ template <int NS, bool COH = false>
 __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D, CloseShared &sh)
 {
     const DevCell &C = a.cells[cell];
-    const int ks = static_cast<int>(D.step & 1);
     const int tid = threadIdx.x;
-    // the parameters and derived constants, loaded now and stored to LDS once the
-    // reduction slots are free
-    constexpr int kPW = sizeof(gw_params_t) / 8, kCW = sizeof(CellConsts) / 8;
-    constexpr int kStage = (kPW + kCW + NS - 1) / NS;
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+    // the cell's status, step, time and potential, read before the reduction's barriers
+    const int status = D.status;
+    const bool clamp = D.clamp != 0;
+    const int64_t step_new = D.step + 1;
+    const int ks = static_cast<int>((step_new - 1) & 1);
+    const double t_old = D.t;
+    const double V_old = D.g.V;
+    const double t_new = t_old + a.dt;
+    // the stimulus of the new time, on the last wave (the first leads the
+    // serial part below)
+    const int istim_tid = (nw - 1) * kWave;
+    double Istim = 0.0;
+    if (tid == istim_tid && status == 0) Istim = protocol_istim(C, t_new, step_new);
+    // the parameters, derived constants and observables, loaded now and stored
+    // to LDS once the reduction slots are free
+    constexpr int kPW = sizeof(gw_params_t) / 8, kCW = sizeof(CellConsts) / 8, kOW = sizeof(gw_cell_obs_t) / 8;
+    constexpr int kStage = (kPW + kCW + kOW + NS - 1) / NS;
     unsigned long long stage[kStage];
 #pragma unroll
     for (int r = 0; r < kStage; ++r) {
@@ -519,6 +548,8 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         if (tid < NS) {
             if (w < kPW) stage[r] = reinterpret_cast<const unsigned long long *>(&C.p)[w];
             else if (w < kPW + kCW) stage[r] = reinterpret_cast<const unsigned long long *>(&C.c)[w - kPW];
+            else if (w < kPW + kCW + kOW)
+                stage[r] = reinterpret_cast<const unsigned long long *>(&D.obs)[w - kPW - kCW];
         }
     }
     // this thread's global and its increment, read now for globals_apply below
@@ -527,38 +558,42 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         g_in = reinterpret_cast<const double *>(&D.g)[tid];
         dg_in = reinterpret_cast<const double *>(&D.dg)[tid];
     }
+    // the block partials: read by every wave (by the first alone where they
+    // are read coherently, after its wait for them), summed within each wave,
+    // then across the waves; the sums are exact, so the order does not matter
     Partial mine;
     partial_zero(mine);
-    if (tid < NS) {
+    {
         const Partial *src = a.partials + ks * a.nblocks + C.blk0[a.shape];
         const int nb = C.nblk[a.shape];
         if constexpr (COH) {
-            for (int q = tid; q < nb; q += NS) {
-                Partial p;
-                unsigned long long *pw = reinterpret_cast<unsigned long long *>(&p);
-                const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(src + q);
-                for (int t = 0; t < static_cast<int>(sizeof(Partial) / 8); ++t)
-                    pw[t] = __hip_atomic_load(sw + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                partial_add(mine, p);
+            if (tid < NS) {
+                for (int q = tid; q < nb; q += NS) {
+                    Partial p;
+                    unsigned long long *pw = reinterpret_cast<unsigned long long *>(&p);
+                    const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(src + q);
+                    for (int t = 0; t < static_cast<int>(sizeof(Partial) / 8); ++t)
+                        pw[t] = __hip_atomic_load(sw + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    partial_add(mine, p);
+                }
             }
         } else {
-            // two partials in flight per thread (the sums are exact, so the order does not matter)
+            // two partials in flight per thread
+            const int nt = static_cast<int>(blockDim.x);
             int q = tid;
-            for (; q + NS < nb; q += 2 * NS) {
-                const Partial p0 = src[q], p1 = src[q + NS];
+            for (; q + nt < nb; q += 2 * nt) {
+                const Partial p0 = src[q], p1 = src[q + nt];
                 partial_add(mine, p0);
                 partial_add(mine, p1);
             }
             if (q < nb) partial_add(mine, src[q]);
         }
     }
-    reduce_slots<NS>(sh.red, mine, tid, tid < NS);
-    const Partial tot = sh.red[0];
-    const int status = D.status;
-    const bool clamp = D.clamp != 0;
-    const double t_old = D.t;
-    const double V_old = D.g.V;
-    const int64_t step_new = D.step + 1;
+    partial_wave_sum(mine);
+    if ((tid & (kWave - 1)) == 0) sh.red[tid / kWave] = mine;
+    __syncthreads();
+    Partial tot = sh.red[0];
+    for (int w = 1; w < nw; ++w) partial_add(tot, sh.red[w]);
     __syncthreads();   // the reduction slots are reused below
     if (tid == 0 && a.iters_total) atomicAdd(a.iters_total, static_cast<unsigned long long>(tot.iters));
     if (status != 0) return;
@@ -576,11 +611,12 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         if (tid < NS) {
             if (w < kPW) reinterpret_cast<unsigned long long *>(&L.p)[w] = stage[r];
             else if (w < kPW + kCW) reinterpret_cast<unsigned long long *>(&L.c)[w - kPW] = stage[r];
+            else if (w < kPW + kCW + kOW) reinterpret_cast<unsigned long long *>(&L.obs)[w - kPW - kCW] = stage[r];
         }
     }
+    if (nw > 1 && tid == istim_tid) L.Istim = Istim;
     const FluxSums S{fx_to_double(tot.jlcc), fx_to_double(tot.jxfer), fx_to_double(tot.jtr), static_cast<double>(tot.clch)};
     const double jrel = fx_to_double(tot.jrel);
-    const double t_new = t_old + a.dt;
     // globals_apply (GW.tpp:252-271), one global per thread; V is held under a
     // clamp and then set to the clamp potential of the new time (prepare_step)
     if (tid < 36) {
@@ -592,7 +628,14 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     __syncthreads();
     const gw_globals_t &g = *reinterpret_cast<const gw_globals_t *>(L.g);
     const double VF_RT = g.V * L.c.F_RT;
-    if (tid == 0) gd_args(L.p, g, VF_RT, L.x, L.py);
+    // the arguments of the state-dependent transcendentals: the two halves on
+    // two waves when the block has them
+    if (tid == 0) {
+        gd_args_lo(L.p, g, VF_RT, L.x);
+        if (blockDim.x < 2 * kWave) gd_args_hi(L.p, g, L.x, L.py);
+    } else if (tid == kWave) {
+        gd_args_hi(L.p, g, L.x, L.py);
+    }
     __syncthreads();
     if (tid < GD_N) {
         const double v = gd_eval(tid, L.x[tid], tid >= GD_POW0 ? L.py[tid - GD_POW0] : 0.0);
@@ -603,13 +646,13 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     // gates and the three K-channel chains on the block's second and third
     // waves when it has them (they read only the parameters, the globals and
     // the evaluated exponentials), on lane 0 otherwise
-    Currents cur;
-    gw_globals_t dg;
-    const int nw = static_cast<int>(blockDim.x) / kWave;
     if (tid == 0) {
-        const double Istim = protocol_istim(C, t_new, step_new);
+        Currents cur;
+        gw_globals_t dg;
+        if (nw > 1) Istim = L.Istim;
         const int rc = gd_finish_main(L.p, L.c, g, VF_RT, L.x, a.dt, Istim, S, dg, cur);
         if (rc != GW_OK) D.status = rc;
+        L.ICaL = cur.ICaL;
         if (nw == 1) {
             double gates[4];
             gd_gates(L.x, g, a.dt, gates);
@@ -622,7 +665,8 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
             gd_chain_kv(L.p, L.x, g, a.dt, 1, dg.Kv43);
         }
         D.Istim = Istim;
-        D.dg = dg;
+        if (nw == 1) D.dg = dg;
+        else for (int k = 0; k < kGd0; ++k) L.dg0[k] = reinterpret_cast<const double *>(&dg)[k];
         // lookahead_consts: the potential and concentrations the step after starts from
         double V = g.V, Cai = g.Cai, CaNSR = g.CaNSR;
         if (!a.frozen) {
@@ -647,18 +691,52 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         gd_chain_kr(L.p, L.x, g, a.dt, L.ch + 4);       // IKr
     }
     __syncthreads();
-    if (nw > 1 && tid < 29) reinterpret_cast<double *>(&D.dg.m)[tid] = L.ch[tid];
+    if (nw > 1 && tid < 36) reinterpret_cast<double *>(&D.dg)[tid] = tid < kGd0 ? L.dg0[tid] : L.ch[tid - kGd0];
     if (tid < SC_N) {
         const double v = gw::exp(L.sx[tid]);
         L.sx[tid] = v;
     }
     __syncthreads();
     if (tid < 36) reinterpret_cast<double *>(&D.g)[tid] = L.g[tid];
-    if (tid != 0) return;
+    // the SSA constants of step_new + 1: the LCC rate table one state per lane,
+    // the rest on lane 0 below
     const int slot = static_cast<int>((step_new + 1) & 1);
-    step_consts_x(D.k[slot], L.p, L.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
-                  static_cast<uint32_t>((step_new + 1) & 0xffff));
-    step_consts_f32(D.kf[slot], D.k[slot]);
+    if (tid < 13) {
+        const LccRow w = lcc_row(tid, L.p, L.c, L.sx);
+        lcc_row_store(D.k[slot].lcc, tid, w);
+        lcc_row_store(D.kf[slot].lcc, tid, w);
+    }
+    // observables and trace, on the second wave when the block has one
+    if (tid == (nw > 1 ? kWave : 0)) {
+        const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
+        gw_cell_obs_t o = L.obs;
+        obs_update(o, V_old, g.V, t_old, a.dt, g.Cai, L.ICaL, open_frac, jrel * L.c.CRU_factor);
+        D.obs = o;
+        if (a.trace_every > 0 && (step_new % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
+            double *row = a.trace + (D.trace_rows * a.ncells + cell) * 4;
+            row[0] = g.V;
+            row[1] = g.Cai;
+            row[2] = L.ICaL;
+            row[3] = open_frac;
+            D.trace_rows += 1;
+        }
+    }
+    if (tid != 0) return;
+    {   // formed in registers, then stored in both precisions
+        StepConsts k;
+        step_consts_scalars(k, L.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
+                            static_cast<uint32_t>((step_new + 1) & 0xffff));
+        StepConsts &K = D.k[slot];
+        K.Cai = k.Cai;
+        K.CaNSR = k.CaNSR;
+        K.inact0 = k.inact0;
+        K.inact1 = k.inact1;
+        K.JLCC_mult = k.JLCC_mult;
+        K.JLCC_exp = k.JLCC_exp;
+        K.step_lo = k.step_lo;
+        K._pad = 0;
+        step_consts_f32_scalars(D.kf[slot], k);
+    }
     D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
     D.sums[4] = static_cast<double>(tot.open); D.sums[5] = jrel;
     D.t = t_new;
@@ -671,16 +749,6 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     // changes from a value above step_new to step_new + 1, so every block of
     // that launch runs the step and none runs a later one.
     if (D.status != 0) D.stop_step = step_new + 1;
-    const double open_frac = static_cast<double>(tot.open) / (20.0 * C.n_cru);
-    obs_update(D.obs, V_old, g.V, t_old, a.dt, g.Cai, cur.ICaL, open_frac, jrel * L.c.CRU_factor);
-    if (a.trace_every > 0 && (step_new % a.trace_every) == 0 && D.trace_rows < a.trace_cap) {
-        double *row = a.trace + (D.trace_rows * a.ncells + cell) * 4;
-        row[0] = g.V;
-        row[1] = g.Cai;
-        row[2] = cur.ICaL;
-        row[3] = open_frac;
-        D.trace_rows += 1;
-    }
 }
 
 // One launch per Euler step: the first `nclose` blocks close step ssa_step-1

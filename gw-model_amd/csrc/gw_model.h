@@ -94,16 +94,13 @@ GW_HD void sc_args(double V, double F_RT, double *arg)
     arg[SC_EXPM] = -(V * F_RT);
 }
 
-// Per-step constants of GW_model::SSA (GW.tpp:216-221) and the LCC rate table
-// of update_LCC_rates (GW_utils.tpp:118-186), from the exponentials X[SC_*]
-// of sc_args. At V == 0 the multiplier takes its analytic limit
-// JLCC_const / 2 instead of 0/0.
-GW_HD void step_consts_x(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, const double *X,
-                         double Cai, double CaNSR, uint32_t step_lo)
+// Per-step constants of GW_model::SSA (GW.tpp:216-221) from the exponentials
+// X[SC_*] of sc_args, without the LCC rate table. At V == 0 the multiplier
+// takes its analytic limit JLCC_const / 2 instead of 0/0.
+GW_HD void step_consts_scalars(StepConsts &k, const CellConsts &c, double V, const double *X, double Cai, double CaNSR,
+                               uint32_t step_lo)
 {
     const double expmVF_RT = X[SC_EXPM];
-    const double al = 2.0 * X[SC_AL];
-    const double be = 0.0882 * X[SC_BE];
     const double yinf = 0.4 / (1.0 + X[SC_YINF]) + 0.6;
     const double tau = 340.0 / (1.0 + X[SC_TAU]) + 60.0;
     k.Cai = Cai;
@@ -114,31 +111,63 @@ GW_HD void step_consts_x(StepConsts &k, const gw_params_t &p, const CellConsts &
     k.JLCC_mult = (V == 0.0) ? c.JLCC_const / 2.0 : c.JLCC_const * V * c.F_RT / (k.JLCC_exp - 1);
     k.step_lo = step_lo;
     k._pad = 0;
-    // r0, r1, r2 of each state; the CaSS-proportional rate carries its coefficient in Q.
-    double r0[13], r1[13], r2[13], q[13];
-    for (int s = 0; s < 13; ++s) { r0[s] = 0; r1[s] = 0; r2[s] = 0; q[s] = 0; }
-    r0[1] = 4 * al;           q[1] = p.gamma0;          // r1 = gamma0 CaSS
-    r0[2] = be;               r1[2] = 3 * al;           q[2] = c.gamma0a;   // r2 = gamma0 a CaSS
-    r0[3] = 2 * be;           r1[3] = 2 * al;           q[3] = c.gamma0a2;
-    r0[4] = 3 * be;           r1[4] = al;               q[4] = c.gamma0a3;
-    r0[5] = 4 * be;           r1[5] = p.f;              q[5] = c.gamma0a4;
-    r0[6] = p.g;
-    r0[7] = p.omega;          r1[7] = 4 * al * p.a;
-    r0[8] = c.omega_b;        r1[8] = be * c.binv;      r2[8] = 3 * al * p.a;
-    r0[9] = c.omega_b2;       r1[9] = 2 * be * c.binv;  r2[9] = 2 * al * p.a;
-    r0[10] = c.omega_b3;      r1[10] = 3 * be * c.binv; r2[10] = al * p.a;
-    r0[11] = c.omega_b4;      r1[11] = 4 * be * c.binv; r2[11] = p.f1;
-    r0[12] = p.g1;
-    for (int s = 0; s < 13; ++s) {
-        k.lcc.R0[s] = r0[s];
-        k.lcc.R1[s] = r1[s];
-        k.lcc.Q[s] = q[s];
-        // state 1 carries its CaSS term in r1: (r0 + g0 CaSS) + 0 == P + Q CaSS with P = r0.
-        // states 2..5 carry it in r2: (r0 + r1) + Q CaSS. Others: (r0 + r1) + r2 + 0.
-        if (s == 1) k.lcc.P[s] = r0[s];
-        else if (s >= 2 && s <= 5) k.lcc.P[s] = r0[s] + r1[s];
-        else k.lcc.P[s] = r0[s] + r1[s] + r2[s];
+}
+
+struct LccRow {
+    double P, Q, R0, R1;
+};
+
+// Row s of the LCC rate table of update_LCC_rates (GW_utils.tpp:118-186):
+// r0, r1, r2 of state s, the CaSS-proportional rate's coefficient in Q.
+// State 1 carries its CaSS term in r1: (r0 + g0 CaSS) + 0 == P + Q CaSS with
+// P = r0; states 2..5 carry it in r2: (r0 + r1) + Q CaSS; the others
+// (r0 + r1) + r2 + 0.
+GW_HD LccRow lcc_row(int s, const gw_params_t &p, const CellConsts &c, const double *X)
+{
+    const double al = 2.0 * X[SC_AL];
+    const double be = 0.0882 * X[SC_BE];
+    double r0 = 0, r1 = 0, r2 = 0, q = 0;
+    switch (s) {
+    case 1: r0 = 4 * al; q = p.gamma0; break;
+    case 2: r0 = be; r1 = 3 * al; q = c.gamma0a; break;
+    case 3: r0 = 2 * be; r1 = 2 * al; q = c.gamma0a2; break;
+    case 4: r0 = 3 * be; r1 = al; q = c.gamma0a3; break;
+    case 5: r0 = 4 * be; r1 = p.f; q = c.gamma0a4; break;
+    case 6: r0 = p.g; break;
+    case 7: r0 = p.omega; r1 = 4 * al * p.a; break;
+    case 8: r0 = c.omega_b; r1 = be * c.binv; r2 = 3 * al * p.a; break;
+    case 9: r0 = c.omega_b2; r1 = 2 * be * c.binv; r2 = 2 * al * p.a; break;
+    case 10: r0 = c.omega_b3; r1 = 3 * be * c.binv; r2 = al * p.a; break;
+    case 11: r0 = c.omega_b4; r1 = 4 * be * c.binv; r2 = p.f1; break;
+    case 12: r0 = p.g1; break;
+    default: break;
     }
+    LccRow w;
+    w.R0 = r0;
+    w.R1 = r1;
+    w.Q = q;
+    if (s == 1) w.P = r0;
+    else if (s >= 2 && s <= 5) w.P = r0 + r1;
+    else w.P = r0 + r1 + r2;
+    return w;
+}
+
+template <class R>
+GW_HD void lcc_row_store(LccTableT<R> &t, int s, const LccRow &w)
+{
+    t.P[s] = static_cast<R>(w.P);
+    t.Q[s] = static_cast<R>(w.Q);
+    t.R0[s] = static_cast<R>(w.R0);
+    t.R1[s] = static_cast<R>(w.R1);
+}
+
+// Per-step constants of GW_model::SSA and the LCC rate table, from the
+// exponentials X[SC_*] of sc_args.
+GW_HD void step_consts_x(StepConsts &k, const gw_params_t &p, const CellConsts &c, double V, const double *X,
+                         double Cai, double CaNSR, uint32_t step_lo)
+{
+    step_consts_scalars(k, c, V, X, Cai, CaNSR, step_lo);
+    for (int s = 0; s < 13; ++s) lcc_row_store(k.lcc, s, lcc_row(s, p, c, X));
 }
 
 // step_consts_x with its exponentials evaluated in turn (exp(-V F/RT) given).
@@ -161,7 +190,7 @@ struct StepConstsF {
     LccTableF lcc;
 };
 
-GW_HD void step_consts_f32(StepConstsF &f, const StepConsts &k)
+GW_HD void step_consts_f32_scalars(StepConstsF &f, const StepConsts &k)
 {
     f.Cai_d = k.Cai;
     f.CaNSR_d = k.CaNSR;
@@ -171,12 +200,12 @@ GW_HD void step_consts_f32(StepConstsF &f, const StepConsts &k)
     f.inact1 = static_cast<float>(k.inact1);
     f.JLCC_mult = static_cast<float>(k.JLCC_mult);
     f.JLCC_exp = static_cast<float>(k.JLCC_exp);
-    for (int s = 0; s < 13; ++s) {
-        f.lcc.P[s] = static_cast<float>(k.lcc.P[s]);
-        f.lcc.Q[s] = static_cast<float>(k.lcc.Q[s]);
-        f.lcc.R0[s] = static_cast<float>(k.lcc.R0[s]);
-        f.lcc.R1[s] = static_cast<float>(k.lcc.R1[s]);
-    }
+}
+
+GW_HD void step_consts_f32(StepConstsF &f, const StepConsts &k)
+{
+    step_consts_f32_scalars(f, k);
+    for (int s = 0; s < 13; ++s) lcc_row_store(f.lcc, s, LccRow{k.lcc.P[s], k.lcc.Q[s], k.lcc.R0[s], k.lcc.R1[s]});
 }
 
 // ---------------------------------------------------------------------------
@@ -214,7 +243,9 @@ GW_HD double gd_eval(int slot, double arg, double y)
     return slot < GD_LOG0 ? gw::exp(arg) : slot < GD_POW0 ? gw::log(arg) : gw::pow(arg, y);
 }
 
-GW_HD void gd_args(const gw_params_t &p, const gw_globals_t &g, double VF_RT, double *arg, double *py)
+// The arguments of the slots before GD_KR0 (gd_args_lo) and from it on
+// (gd_args_hi); gd_args is both.
+GW_HD void gd_args_lo(const gw_params_t &p, const gw_globals_t &g, double VF_RT, double *arg)
 {
     const double V = g.V;
     arg[GD_EXPM] = -VF_RT;
@@ -238,6 +269,11 @@ GW_HD void gd_args(const gw_params_t &p, const gw_globals_t &g, double VF_RT, do
     arg[GD_XKS] = -(V - 24.7) / 13.6;
     arg[GD_TXKS1] = -0.148 * (V - 10.0);
     arg[GD_TXKS2] = 0.0687 * (V - 10.0);
+}
+
+GW_HD void gd_args_hi(const gw_params_t &p, const gw_globals_t &g, double *arg, double *py)
+{
+    const double V = g.V;
     arg[GD_KR0] = 0.0272 * V;
     arg[GD_KR1] = -0.0431 * V;
     arg[GD_KR2] = 0.0262 * V;
@@ -262,6 +298,12 @@ GW_HD void gd_args(const gw_params_t &p, const gw_globals_t &g, double VF_RT, do
     py[1] = p.Hf;
     arg[GD_PR] = g.CaNSR / p.Kmr;
     py[2] = p.Hr;
+}
+
+GW_HD void gd_args(const gw_params_t &p, const gw_globals_t &g, double VF_RT, double *arg, double *py)
+{
+    gd_args_lo(p, g, VF_RT, arg);
+    gd_args_hi(p, g, arg, py);
 }
 
 // Nernst potential (common.hpp:70-75) from lg = log(Xo / Xi); the reference
