@@ -532,10 +532,12 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     const double V_old = D.g.V;
     const double t_new = t_old + a.dt;
     // the stimulus of the new time, on the last wave (the first leads the
-    // serial part below)
+    // serial part below), formed while the partials below are loading
     const int istim_tid = (nw - 1) * kWave;
     double Istim = 0.0;
-    if (tid == istim_tid && status == 0) Istim = protocol_istim(C, t_new, step_new);
+    if constexpr (COH) {
+        if (tid == istim_tid && status == 0) Istim = protocol_istim(C, t_new, step_new);
+    }
     // the parameters, derived constants and observables, loaded now and stored
     // to LDS once the reduction slots are free
     constexpr int kPW = sizeof(gw_params_t) / 8, kCW = sizeof(CellConsts) / 8, kOW = sizeof(gw_cell_obs_t) / 8;
@@ -578,10 +580,17 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
                 }
             }
         } else {
-            // two partials in flight per thread
+            // the thread's first partial (the only one where the cell has no
+            // more blocks than the close has threads) is in flight while the
+            // stimulus is formed; the rest two at a time
             const int nt = static_cast<int>(blockDim.x);
             int q = tid;
-            for (; q + nt < nb; q += 2 * nt) {
+            Partial first;
+            const bool has_first = q < nb;
+            if (has_first) first = src[q];
+            if (tid == istim_tid && status == 0) Istim = protocol_istim(C, t_new, step_new);
+            if (has_first) partial_add(mine, first);
+            for (q += nt; q + nt < nb; q += 2 * nt) {
                 const Partial p0 = src[q], p1 = src[q + nt];
                 partial_add(mine, p0);
                 partial_add(mine, p1);
@@ -698,13 +707,29 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     }
     __syncthreads();
     if (tid < 36) reinterpret_cast<double *>(&D.g)[tid] = L.g[tid];
-    // the SSA constants of step_new + 1: the LCC rate table one state per lane,
-    // the rest on lane 0 below
+    // the SSA constants of step_new + 1: the LCC rate table one state per lane
     const int slot = static_cast<int>((step_new + 1) & 1);
     if (tid < 13) {
         const LccRow w = lcc_row(tid, L.p, L.c, L.sx);
         lcc_row_store(D.k[slot].lcc, tid, w);
         lcc_row_store(D.kf[slot].lcc, tid, w);
+    }
+    // the rest of the constants of step_new + 1, formed in registers and
+    // stored in both precisions: on the third wave when the block has one
+    if (tid == (nw > 2 ? 2 * kWave : 0)) {
+        StepConsts k;
+        step_consts_scalars(k, L.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
+                            static_cast<uint32_t>((step_new + 1) & 0xffff));
+        StepConsts &K = D.k[slot];
+        K.Cai = k.Cai;
+        K.CaNSR = k.CaNSR;
+        K.inact0 = k.inact0;
+        K.inact1 = k.inact1;
+        K.JLCC_mult = k.JLCC_mult;
+        K.JLCC_exp = k.JLCC_exp;
+        K.step_lo = k.step_lo;
+        K._pad = 0;
+        step_consts_f32_scalars(D.kf[slot], k);
     }
     // observables and trace, on the second wave when the block has one
     if (tid == (nw > 1 ? kWave : 0)) {
@@ -722,21 +747,6 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
         }
     }
     if (tid != 0) return;
-    {   // formed in registers, then stored in both precisions
-        StepConsts k;
-        step_consts_scalars(k, L.c, L.lookahead[0], L.sx, L.lookahead[1], L.lookahead[2],
-                            static_cast<uint32_t>((step_new + 1) & 0xffff));
-        StepConsts &K = D.k[slot];
-        K.Cai = k.Cai;
-        K.CaNSR = k.CaNSR;
-        K.inact0 = k.inact0;
-        K.inact1 = k.inact1;
-        K.JLCC_mult = k.JLCC_mult;
-        K.JLCC_exp = k.JLCC_exp;
-        K.step_lo = k.step_lo;
-        K._pad = 0;
-        step_consts_f32_scalars(D.kf[slot], k);
-    }
     D.sums[0] = S.JLCC; D.sums[1] = S.Jxfer; D.sums[2] = S.Jtr; D.sums[3] = S.ClCh;
     D.sums[4] = static_cast<double>(tot.open); D.sums[5] = jrel;
     D.t = t_new;
