@@ -1149,19 +1149,46 @@ int gwo_global_rhs(const gw_params_t *p, int nCRU, const gw_globals_t *g, double
 }
 
 /* The four sums the derivatives read (GW.tpp:188,192,197-198), formed as D2. */
+/* the exact sum of two accumulators (128-bit two's complement) */
+static void fx_merge(gwo_fx *a, const gwo_fx *b)
+{
+    u128 acc = ((u128)a->hi << 64) | a->lo;
+    acc += ((u128)b->hi << 64) | b->lo;
+    a->lo = (uint64_t)acc;
+    a->hi = (uint64_t)(acc >> 64);
+}
+
+/* The sums are exact, so they do not depend on the order of the CRUs: with
+   OpenMP each thread sums its share and the shares are added after. */
 static int flux_sums(const gwo_model *m, double *sJLCC, double *sJxfer, double *sJtr, double *sClCh, double *sOpen)
 {
     gwo_fx aL = {0, 0}, aX = {0, 0}, aT = {0, 0};
     int flag = 0;
     int64_t clch = 0, open = 0;
-    for (int i = 0; i < m->n; ++i) {
-        const double *L = m->JLCC + 4 * i, *X = m->Jxfer + 4 * i;
-        gwo_fx_add(&aL, ((L[0] + L[1]) + L[2]) + L[3], &flag);
-        gwo_fx_add(&aX, ((X[0] + X[1]) + X[2]) + X[3], &flag);
-        gwo_fx_add(&aT, m->Jtr[i], &flag);
-        for (int j = 0; j < 4; ++j) {
-            clch += m->ClCh[4 * i + j];
-            open += m->RyR[24 * i + 6 * j + 2] + m->RyR[24 * i + 6 * j + 3];
+#pragma omp parallel
+    {
+        gwo_fx tL = {0, 0}, tX = {0, 0}, tT = {0, 0};
+        int tflag = 0;
+        int64_t tclch = 0, topen = 0;
+#pragma omp for schedule(static) nowait
+        for (int i = 0; i < m->n; ++i) {
+            const double *L = m->JLCC + 4 * i, *X = m->Jxfer + 4 * i;
+            gwo_fx_add(&tL, ((L[0] + L[1]) + L[2]) + L[3], &tflag);
+            gwo_fx_add(&tX, ((X[0] + X[1]) + X[2]) + X[3], &tflag);
+            gwo_fx_add(&tT, m->Jtr[i], &tflag);
+            for (int j = 0; j < 4; ++j) {
+                tclch += m->ClCh[4 * i + j];
+                topen += m->RyR[24 * i + 6 * j + 2] + m->RyR[24 * i + 6 * j + 3];
+            }
+        }
+#pragma omp critical(gwo_flux_sums)
+        {
+            fx_merge(&aL, &tL);
+            fx_merge(&aX, &tX);
+            fx_merge(&aT, &tT);
+            flag |= tflag;
+            clch += tclch;
+            open += topen;
         }
     }
     *sJLCC = gwo_fx_to_double(&aL);
