@@ -1159,13 +1159,15 @@ static void fx_merge(gwo_fx *a, const gwo_fx *b)
 }
 
 /* The sums are exact, so they do not depend on the order of the CRUs: with
-   OpenMP each thread sums its share and the shares are added after. */
+   OpenMP each thread sums its share of a large cell and the shares are added
+   after (a small cell is summed by one thread: a parallel region per step
+   would cost more than it saves). */
 static int flux_sums(const gwo_model *m, double *sJLCC, double *sJxfer, double *sJtr, double *sClCh, double *sOpen)
 {
     gwo_fx aL = {0, 0}, aX = {0, 0}, aT = {0, 0};
     int flag = 0;
     int64_t clch = 0, open = 0;
-#pragma omp parallel
+#pragma omp parallel if (m->n >= 4096)
     {
         gwo_fx tL = {0, 0}, tX = {0, 0}, tT = {0, 0};
         int tflag = 0;

@@ -374,11 +374,11 @@ def test_ssa_realises_the_rate_table():
         assert abs(ccnt[state] - expect) < 4 * math.sqrt(expect) + 2, (state, ccnt[state], expect)
 
 
-def test_parallel_restatement_matches_the_serial_one():
-    """The OpenMP build of the restatement (per-CRU streams, the flux sums exact
-    and summed per thread) gives the serial build's bits: 500 CRUs through the
-    paced upstroke and the start of release, 3,000 steps."""
-    n, steps = 500, 3000
+@pytest.mark.parametrize("n,steps", [(500, 3000), (4200, 1500)], ids=["small", "sums-per-thread"])
+def test_parallel_restatement_matches_the_serial_one(n, steps):
+    """The OpenMP build of the restatement (per-CRU streams; the exact flux sums
+    of a cell of 4,096 CRUs or more summed per thread) gives the serial build's
+    bits through the paced upstroke and the start of release."""
     ist = np.zeros(steps)
     ist[:1000] = 100.0
     runs = []
