@@ -532,7 +532,8 @@ __device__ __forceinline__ void close_cell(const KArgs &a, int cell, CellDyn &D,
     const double V_old = D.g.V;
     const double t_new = t_old + a.dt;
     // the stimulus of the new time, on the last wave (the first leads the
-    // serial part below), formed while the partials below are loading
+    // serial part below): formed below while that wave's first block partial
+    // is loading, or here where the partials are read coherently
     const int istim_tid = (nw - 1) * kWave;
     double Istim = 0.0;
     if constexpr (COH) {
