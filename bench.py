@@ -434,6 +434,41 @@ def measure(capi, precision, args, total_crus):
     }
 
 
+LATENCY_BATCHES = ((1, 64), (1, 1250), (1, 12500), (4, 12500), (16, 12500))
+
+
+def latency_run(capi, args):
+    """Per-step time of small batches (a cell of the reference's sizes and a
+    few of them) in both forms at the default launch shape: seeded rest state,
+    1 Hz pacing, 1,000 untimed steps, then gw_batch_time_steps over
+    --latency-steps steps."""
+    ctx = capi.Context(0)
+    rows = []
+    for ncells, n in LATENCY_BATCHES:
+        row = {"cells": ncells, "crus": n}
+        for name, prec in (("fp64", capi.PREC_FP64), ("fp32_local", capi.PREC_FP32_LOCAL)):
+            b = capi.Batch(ctx, [capi.make_cell(n, 42 + c, None, pacing(capi)) for c in range(ncells)], precision=prec)
+            b.set_mode(False, False)
+            b.step(1000)
+            ms, _, _ = b.time_steps(args.latency_steps)
+            b.close()
+            row[f"{name}_us_per_step"] = ms / args.latency_steps * 1e3
+        rows.append(row)
+        print(f"bench: {ncells} x {n} CRUs: FP64 {row['fp64_us_per_step']:.1f} us, "
+              f"FP32-local {row['fp32_local_us_per_step']:.1f} us per step", file=sys.stderr, flush=True)
+    ctx.close()
+    return {
+        "metric": "time per Euler step of small batches, 1 GPU",
+        "unit": "us/step",
+        "higher_is_better": False,
+        "data": "synthetic: seeded rest state, reference default parameters, 1 Hz pacing",
+        "config": {"workload": "1 x 64, 1 x 1,250, 1 x 12,500, 4 x 12,500 and 16 x 12,500 CRUs, dt 1e-3 ms, "
+                               f"{args.latency_steps} steps timed after 1,000 untimed, default launch shape",
+                   "steps": args.latency_steps},
+        "batches": rows,
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,6 +508,10 @@ def main(argv=None):
     ap.add_argument("--alternans-beats", type=int, default=0, metavar="N",
                     help="only run the FP32-local batch from rest through N whole beats and report the per-cell "
                          "APD90 alternans from the on-device tracker (one JSON line), instead of the throughput bench")
+    ap.add_argument("--latency", action="store_true",
+                    help="only time one step of small batches (1 to 16 cells) in both forms (one JSON line), "
+                         "instead of the throughput bench")
+    ap.add_argument("--latency-steps", type=int, default=5000, help="timed steps per batch of --latency")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -493,6 +532,11 @@ def main(argv=None):
         return 0
     if args.alternans_beats > 0:
         out = alternans_run(capi, args)
+        if rank == 0:
+            print(json.dumps(out), flush=True)
+        return 0
+    if args.latency:
+        out = latency_run(capi, args)
         if rank == 0:
             print(json.dumps(out), flush=True)
         return 0

@@ -103,7 +103,8 @@ def test_whole_beat_modes_and_steps_per_launch_parse_before_the_gpu(bench):
     """The whole-beat modes and the steps-per-launch choice are accepted
     options; more than one GPU is still refused before any GPU work."""
     for extra in (["--ensemble-beats", "5", "--workload", "seeds64"], ["--alternans-beats", "3"],
-                  ["--steps-per-launch", "0"], ["--steps-per-launch", "2"], ["--steps-per-launch", "1"]):
+                  ["--steps-per-launch", "0"], ["--steps-per-launch", "2"], ["--steps-per-launch", "1"],
+                  ["--latency"], ["--latency", "--latency-steps", "100"]):
         assert bench.main(["--gpus", "2"] + extra) == 2
     with pytest.raises(SystemExit):
         bench.main(["--steps-per-launch", "3"])
